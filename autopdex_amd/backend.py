@@ -1,0 +1,169 @@
+"""Device problem plans for the 'hip' backend.
+
+A DeviceProblem owns one native plan (mesh, DOF layout, Dirichlet mask,
+reduced CSR pattern, element colours) and the device copies of the nodal
+fields it needs.  Plans are cached per (coordinates, connectivities, mask,
+models): repeated solver calls on the same mesh -- Newton load steps, new
+coefficient fields -- reuse the plan and only update parameters.
+
+PyTorch provides device memory and the current HIP stream; all compute is in
+libautopdex_amd.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from . import models as M
+from . import utility
+
+
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("autopdex_amd 'hip' backend needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def current_stream():
+    torch = torch_cuda()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _field_key(dofs):
+    if isinstance(dofs, dict):
+        if len(dofs) != 1:
+            raise NotImplementedError("autopdex_amd supports a single DOF field per problem")
+        return next(iter(dofs))
+    return None
+
+
+def _pick(x, key):
+    return x[key] if (key is not None and isinstance(x, dict)) else x
+
+
+class DeviceProblem:
+    def __init__(self, dofs, settings, static_settings):
+        torch = torch_cuda()
+        self.key = _field_key(dofs)
+        u = np.asarray(_pick(dofs, self.key))
+        self.dofs_shape = u.shape
+        coords = np.ascontiguousarray(_pick(settings["node coordinates"], self.key), dtype=np.float64)
+        self.n_nodes, self.dim = coords.shape
+        self.d = 1 if u.ndim == 1 else u.shape[-1]
+        if u.shape[0] != self.n_nodes:
+            raise ValueError("dofs and node coordinates disagree on the number of nodes")
+        modes = static_settings["assembling mode"]
+        if any(m != "user potential" for m in modes):
+            raise NotImplementedError("the 'hip' backend implements the 'user potential' assembling mode")
+        self.models = list(static_settings["model"])
+        mask = np.ascontiguousarray(utility.dict_flatten(settings["dirichlet dofs"]).astype(np.uint8))
+        if mask.size != self.n_nodes * self.d:
+            raise ValueError("'dirichlet dofs' must have the structure of dofs")
+        plan = C.c_void_p()
+        N.call("apx_plan_create", C.byref(plan), self.dim, self.n_nodes, coords.ctypes.data, self.d,
+               mask.ctypes.data)
+        self.plan = plan
+        self._keep = [coords, mask]
+        self.nodal = {}
+        for i, (model, conn_s) in enumerate(zip(self.models, settings["connectivity"])):
+            if not isinstance(model, (M.UserPotential, M.LineTractionPotential)):
+                raise NotImplementedError(
+                    "the 'hip' backend needs autopdex_amd.models descriptors "
+                    "(mixed_reference_domain_potential of an autopdex_amd integrand, or line_traction_potential)")
+            if model.dofs_per_node(self.dim) != self.d:
+                raise ValueError(f"model of set {i} expects {model.dofs_per_node(self.dim)} DOFs per node")
+            conn = np.ascontiguousarray(_pick(conn_s, self.key), dtype=np.int64)
+            params = np.asarray(model.params(settings, self.dim), dtype=np.float64)
+            sid = C.c_int()
+            N.call("apx_plan_add_set", plan, conn.ctypes.data, conn.shape[0], conn.shape[1], model.kernel,
+                   params.ctypes.data if params.size else None, params.size, C.byref(sid))
+            self._keep.append(conn)
+        N.call("apx_plan_finalize", plan)
+        info = N.PlanInfo()
+        N.call("apx_plan_get_info", plan, C.byref(info))
+        self.info = info
+        self.n_dofs, self.n_free, self.nnz = info.n_dofs, info.n_free, info.nnz
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.update(settings)
+
+    def update(self, settings):
+        """Refresh per-call parameters (material, load multiplier) and nodal fields."""
+        torch = torch_cuda()
+        for i, model in enumerate(self.models):
+            params = np.asarray(model.params(settings, self.dim), dtype=np.float64)
+            if params.size:
+                N.call("apx_plan_set_params", self.plan, i, params.ctypes.data, params.size)
+            fields = model.nodal_fields(settings)
+            if fields is not None:
+                devs = []
+                for w, f in enumerate(fields):
+                    if f.size != self.n_nodes:
+                        raise ValueError("nodal fields must hold one value per node")
+                    t = torch.from_numpy(np.ascontiguousarray(f)).to(self.device)
+                    N.call("apx_plan_bind_nodal_field", self.plan, i, w, C.c_void_p(t.data_ptr()))
+                    devs.append(t)
+                self.nodal[i] = devs
+
+    def bind_nodal_device(self, set_index, g, f):
+        """Bind device tensors (FP64, n_nodes) as the nodal fields of a set (no copy)."""
+        N.call("apx_plan_bind_nodal_field", self.plan, set_index, 0, C.c_void_p(g.data_ptr()))
+        N.call("apx_plan_bind_nodal_field", self.plan, set_index, 1, C.c_void_p(f.data_ptr()))
+        self.nodal[set_index] = [g, f]
+
+    def csr_pattern(self):
+        indptr = np.empty(self.n_free + 1, dtype=np.int32)
+        indices = np.empty(self.nnz, dtype=np.int32)
+        N.call("apx_plan_csr_pattern", self.plan, indptr.ctypes.data, indices.ctypes.data)
+        return indptr, indices
+
+    def free_dofs(self):
+        r2f = np.empty(self.n_free, dtype=np.int32)
+        N.call("apx_plan_free_dofs", self.plan, r2f.ctypes.data)
+        return r2f
+
+    def flat(self, dofs):
+        return np.ascontiguousarray(utility.dict_flatten(dofs), dtype=np.float64)
+
+    def to_device(self, arr, dtype=None):
+        torch = torch_cuda()
+        t = torch.from_numpy(np.ascontiguousarray(arr))
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.to(self.device)
+
+    def __del__(self):
+        try:
+            if getattr(self, "plan", None):
+                N.lib().apx_plan_destroy(self.plan)
+        except Exception:
+            pass
+
+
+_CACHE = []
+_CACHE_MAX = 4
+
+
+def get_problem(dofs, settings, static_settings):
+    key_field = _field_key(dofs)
+    ident = (id(_pick(settings["node coordinates"], key_field)),
+             tuple(id(_pick(c, key_field)) for c in settings["connectivity"]),
+             tuple(id(v) for v in (settings["dirichlet dofs"].values() if isinstance(settings["dirichlet dofs"], dict)
+                                   else [settings["dirichlet dofs"]])),
+             tuple(id(m) for m in static_settings["model"]),
+             np.shape(_pick(dofs, key_field)))
+    for entry in _CACHE:
+        if entry[0] == ident:
+            entry[1].update(settings)
+            return entry[1]
+    prob = DeviceProblem(dofs, settings, static_settings)
+    # the referenced objects are kept alive by the entry so their ids stay unique
+    refs = (settings["node coordinates"], settings["connectivity"], settings["dirichlet dofs"],
+            tuple(static_settings["model"]))
+    _CACHE.insert(0, (ident, prob, refs))
+    del _CACHE[_CACHE_MAX:]
+    return prob
+
+
+def clear_cache():
+    _CACHE.clear()

@@ -1,0 +1,180 @@
+// Element-loop assembly kernels: residual, reduced-CSR tangent values and
+// tangent diagonal for the 'user potential' mode.
+//
+// Replaces jacrev / jacfwd(jacrev) of the element potential under lax.map and
+// the .at[].add scatter (assembler.py:1064-1149, :219-436).
+//
+// Determinism: elements are stored colour-major (no two elements of a colour
+// share a node), one launch per colour in fixed order, and every output entry
+// is updated by a plain read-add-write from exactly one lane per launch.  No
+// floating-point atomics, so every assembly is bitwise reproducible.
+#include "assembly.hpp"
+#include "elements.hpp"
+
+namespace apx {
+
+enum { MODE_RES = 0, MODE_TAN = 1, MODE_DIAG = 2 };
+
+struct AsmArgs {
+  const double* coords;  // SoA, dim x n_nodes
+  int64_t n_nodes;
+  const int32_t* conn;   // colour-major, n_elems x npe
+  int64_t e0, e1;        // element range of one colour
+  const double* u;       // n_dofs
+  ModelParams mp;
+  const int32_t* slots;  // colour-major, n_elems x (npe d)^2
+  double* r;             // residual or diagonal output (n_dofs)
+  void* vals;            // reduced CSR values
+};
+
+template <class M, class T, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_team(AsmArgs A) {
+  constexpr int NPE = M::NPE, DIM = M::DIM, D = M::D, ND = NPE * D;
+  constexpr int EPB = BS / NPE;
+  using QPT = typename M::template QP<T>;
+  __shared__ double sx[EPB][NPE][DIM];
+  __shared__ double su[EPB][NPE][D];
+  __shared__ double sn[EPB][NPE][2];
+  __shared__ QPT sq[EPB][NPE];
+
+  const int le = threadIdx.x / NPE, t = threadIdx.x % NPE;
+  const int64_t e = A.e0 + (int64_t)blockIdx.x * EPB + le;
+  const bool active = e < A.e1;
+  int32_t node = 0;
+  if (active) {
+    node = A.conn[e * NPE + t];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) sx[le][t][k] = A.coords[k * A.n_nodes + node];
+#pragma unroll
+    for (int i = 0; i < D; ++i) su[le][t][i] = A.u[(int64_t)node * D + i];
+    if constexpr (M::NODAL) {
+      sn[le][t][0] = A.mp.nodal[0][node];
+      sn[le][t][1] = A.mp.nodal[1][node];
+    }
+  }
+  __syncthreads();
+  if (active) M::template qp<T>(t, sx[le], su[le], sn[le], A.mp, MODE == MODE_RES, sq[le][t]);
+  __syncthreads();
+  if (!active) return;
+
+  if constexpr (MODE == MODE_RES) {
+    double out[D];
+    M::res_row(t, (const typename M::template QP<double>*)sq[le], A.mp, out);
+#pragma unroll
+    for (int i = 0; i < D; ++i) A.r[(int64_t)node * D + i] += out[i];
+  } else {
+    T out[D][ND];
+    M::template tan_rows<T>(t, sq[le], A.mp, out);
+    if constexpr (MODE == MODE_DIAG) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) A.r[(int64_t)node * D + i] += (double)out[i][t * D + i];
+    } else {
+      T* vals = (T*)A.vals;
+      const int32_t* sl = A.slots + (e * ND + (int64_t)t * D) * ND;
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int B = 0; B < ND; ++B) {
+          const int32_t s = sl[i * ND + B];
+          if (s >= 0) vals[s] += out[i][B];
+        }
+    }
+  }
+}
+
+template <class T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <class T>
+void fill(T* p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, s, p, n, v);
+  APX_LAUNCH_CHECK();
+}
+template void fill<double>(double*, int64_t, double, hipStream_t);
+template void fill<float>(float*, int64_t, float, hipStream_t);
+
+template <class M, class T, int MODE>
+static void run_set(Plan& P, ElemSet& S, const double* u, double* r, void* vals, hipStream_t st) {
+  constexpr int BS = (M::NPE == 8) ? 64 : 128;
+  constexpr int EPB = BS / M::NPE;
+  AsmArgs A;
+  A.coords = P.coords.p;
+  A.n_nodes = P.n_nodes;
+  A.conn = S.conn.p;
+  A.u = u;
+  A.mp = S.params;
+  A.slots = S.slots.p;
+  A.r = r;
+  A.vals = vals;
+  if constexpr (M::NODAL) {
+    APX_REQUIRE(S.params.nodal[0] && S.params.nodal[1], APX_ERR_STATE,
+                "variable-coefficient Poisson set needs nodal fields g and f bound");
+  }
+  for (size_t c = 0; c + 1 < S.colour_offsets.size(); ++c) {
+    A.e0 = S.colour_offsets[c];
+    A.e1 = S.colour_offsets[c + 1];
+    const int64_t ne = A.e1 - A.e0;
+    if (ne <= 0) continue;
+    hipLaunchKernelGGL((k_team<M, T, MODE, BS>), dim3((ne + EPB - 1) / EPB), dim3(BS), 0, st, A);
+    APX_LAUNCH_CHECK();
+  }
+}
+
+template <class T, int MODE>
+static void dispatch(Plan& P, ElemSet& S, const double* u, double* r, void* vals, hipStream_t st) {
+  const int key = S.model * 100 + S.npe * 10 + P.dim;
+  switch (key) {
+    case APX_MODEL_POISSON_SOURCE * 100 + 42:
+      APX_REQUIRE(P.d == 1, APX_ERR_ARG, "Poisson model needs one DOF per node");
+      return run_set<PoissonSource<4, 2>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_POISSON_SOURCE * 100 + 83:
+      APX_REQUIRE(P.d == 1, APX_ERR_ARG, "Poisson model needs one DOF per node");
+      return run_set<PoissonSource<8, 3>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_POISSON_VARK * 100 + 42:
+      APX_REQUIRE(P.d == 1, APX_ERR_ARG, "Poisson model needs one DOF per node");
+      return run_set<PoissonVarK<4, 2>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_POISSON_VARK * 100 + 83:
+      APX_REQUIRE(P.d == 1, APX_ERR_ARG, "Poisson model needs one DOF per node");
+      return run_set<PoissonVarK<8, 3>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_NEO_HOOKE * 100 + 42:
+      APX_REQUIRE(P.d == 2, APX_ERR_ARG, "plane-strain Neo-Hooke needs 2 DOFs per node");
+      return run_set<NeoHooke<4, 2>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_NEO_HOOKE * 100 + 83:
+      APX_REQUIRE(P.d == 3, APX_ERR_ARG, "3D Neo-Hooke needs 3 DOFs per node");
+      return run_set<NeoHooke<8, 3>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_LINE_TRACTION * 100 + 22:
+      APX_REQUIRE(P.d == 2, APX_ERR_ARG, "line traction needs 2 DOFs per node");
+      return run_set<LineTraction, T, MODE>(P, S, u, r, vals, st);
+    default:
+      throw Error(APX_ERR_UNSUPPORTED, "no kernel for model " + std::to_string(S.model) + " with " +
+                                           std::to_string(S.npe) + "-node elements in " +
+                                           std::to_string(P.dim) + "D");
+  }
+}
+
+void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st) {
+  fill<double>(r, P.n_dofs, 0.0, st);
+  for (auto& S : P.sets) dispatch<double, MODE_RES>(P, S, u, r, nullptr, st);
+}
+
+void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st) {
+  if (precision == APX_FP64) {
+    fill<double>((double*)vals, P.nnz, 0.0, st);
+    for (auto& S : P.sets) dispatch<double, MODE_TAN>(P, S, u, nullptr, vals, st);
+  } else {
+    fill<float>((float*)vals, P.nnz, 0.0f, st);
+    for (auto& S : P.sets) dispatch<float, MODE_TAN>(P, S, u, nullptr, vals, st);
+  }
+}
+
+void assemble_diagonal(Plan& P, const double* u, double* diag, hipStream_t st) {
+  fill<double>(diag, P.n_dofs, 0.0, st);
+  for (auto& S : P.sets) dispatch<double, MODE_DIAG>(P, S, u, diag, nullptr, st);
+}
+
+}  // namespace apx

@@ -1,0 +1,80 @@
+// Plan: mesh, DOF layout, reduced CSR pattern and device workspace.
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+
+#include "common.hpp"
+
+namespace apx {
+
+constexpr int kMaxParams = 8;
+
+struct ModelParams {
+  double p[kMaxParams];
+  const double* nodal[2];  // device nodal fields (var-k: g, f)
+};
+
+struct ElemSet {
+  int npe = 0;
+  int model = 0;
+  int64_t n_elems = 0;
+  ModelParams params{};
+  std::vector<int64_t> conn_host;  // original order, n_elems * npe
+  // colour-permuted device data
+  DevBuf<int32_t> conn;            // n_elems * npe, colour-major order
+  DevBuf<int32_t> slots;           // n_elems * (npe*d)^2 reduced CSR slots, -1 = Dirichlet
+  std::vector<int64_t> colour_offsets;  // n_colours + 1
+};
+
+// Timing of a kernel family: HIP events on the caller's stream.
+struct KernelTimer {
+  double total_ms = 0.0;
+  int64_t count = 0;
+  int64_t bytes = 0;  // algorithmic bytes per launch
+};
+
+struct Plan {
+  int dim = 0, d = 0;
+  int64_t n_nodes = 0, n_dofs = 0, n_free = 0, nnz = 0;
+  bool finalized = false;
+  double build_seconds = 0.0;
+
+  std::vector<double> coords_host;   // n_nodes * dim row-major
+  std::vector<uint8_t> dmask_host;   // n_dofs
+  std::vector<ElemSet> sets;
+
+  // host copies of the reduced pattern
+  std::vector<int32_t> indptr_h, indices_h, red2full_h, full2red_h;
+
+  // device data
+  DevBuf<double> coords;       // SoA: dim arrays of n_nodes
+  DevBuf<uint8_t> dmask;       // n_dofs
+  DevBuf<int32_t> full2red;    // n_dofs, -1 for Dirichlet
+  DevBuf<int32_t> red2full;    // n_free
+  DevBuf<int32_t> indptr;      // n_free + 1
+  DevBuf<int32_t> indices;     // nnz
+  DevBuf<int32_t> diag_slot;   // n_free
+
+  // workspace (lazily sized)
+  DevBuf<double> vals64, w64;  // nnz, vectors
+  DevBuf<float> vals32, w32;
+  DevBuf<double> r_full, diag_full;  // n_dofs
+  DevBuf<double> partials;     // reduction partials
+  DevBuf<unsigned int> counters;
+  DevBuf<double> kstate;       // device Krylov scalars
+  double* pinned = nullptr;    // host pinned scalars
+  bool timing = false;
+  std::map<std::string, KernelTimer> timers;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~Plan();
+  int64_t device_bytes() const;
+};
+
+inline int64_t ndofs_elem(const Plan& P, const ElemSet& s) { return (int64_t)s.npe * P.d; }
+
+// plan.hip
+void plan_finalize(Plan& P);
+
+}  // namespace apx

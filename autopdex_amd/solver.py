@@ -1,0 +1,169 @@
+"""solver.solver with the 'hip' solver backend.
+
+Same call and return as the reference (solver.py:41-137):
+    sol, infos = solver(dofs, settings, static_settings, **kwargs)
+with infos = (n_iter, res_norm, diverged) for 'newton' / 'damped newton'.
+
+static_settings keys read here (reference keys unchanged):
+  'solver type'            'newton' | 'damped newton' | 'linear'
+  'solver backend'         must be 'hip'
+  'solver'                 'cg' (Jacobi-PCG) -- the reference 'jax' backend's cg
+  'type of preconditioner' 'jacobi' (default) | 'none' | 'learned'
+  'verbose'                as in the reference
+new keys of this backend:
+  'hip precision'          'fp64' (default): FP64 tangent + FP64 PCG;
+                           'mixed': FP32 tangent + FP32 PCG inner solves, the
+                           Newton residual stays FP64 (iterative refinement)
+  'linear tolerance'       Krylov relative tolerance (default 1e-5, the
+                           default of jax.scipy.sparse.linalg.cg that the
+                           reference's newton mode uses, P7)
+  'linear atol', 'linear maxiter'
+  'newton tolerance mode'  'absolute' (reference, P6) | 'relative'
+                           (tol * ||r_free(u0)||_2)
+Newton semantics (P4-P6) are implemented in the device driver apx_newton
+(solver.py:740-948).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from . import utility
+from .backend import current_stream, get_problem
+
+
+class NewtonInfos(tuple):
+    """(n_iter, res_norm, diverged) plus .details (timings, Krylov counts, history)."""
+
+    def __new__(cls, n_iter, res_norm, diverged, details):
+        obj = super().__new__(cls, (n_iter, res_norm, diverged))
+        obj.details = details
+        return obj
+
+
+def _opt(static_settings, key, default):
+    try:
+        return static_settings[key]
+    except KeyError:
+        return default
+
+
+def _check_backend(static_settings):
+    backend = static_settings["solver backend"]
+    if backend != "hip":
+        raise NotImplementedError(f"autopdex_amd provides the 'hip' solver backend only (got {backend!r})")
+    solver = _opt(static_settings, "solver", "cg")
+    if solver not in ("cg", "fcg"):
+        raise NotImplementedError("the 'hip' backend solves with (flexible) preconditioned CG: 'solver': 'cg'")
+
+
+def _precond(static_settings):
+    p = _opt(static_settings, "type of preconditioner", "jacobi")
+    table = {"jacobi": N.APX_PRECOND_JACOBI, "none": N.APX_PRECOND_NONE, None: N.APX_PRECOND_NONE,
+             "learned": N.APX_PRECOND_LEARNED}
+    if p not in table:
+        raise NotImplementedError(f"preconditioner {p!r} is not available in the 'hip' backend")
+    return table[p]
+
+
+def newton_options(static_settings, newton_tol=1e-8, maxiter=30, damping=1.0, reuse_tangent=None):
+    prec = _opt(static_settings, "hip precision", "fp64")
+    if prec not in ("fp64", "mixed"):
+        raise ValueError("'hip precision' must be 'fp64' or 'mixed'")
+    o = N.NewtonOptions()
+    o.newton_tol = float(newton_tol)
+    o.relative = 1 if _opt(static_settings, "newton tolerance mode", "absolute") == "relative" else 0
+    o.maxiter = int(maxiter)
+    o.damping = float(damping)
+    o.precision = N.APX_FP64 if prec == "fp64" else N.APX_FP32
+    o.precond = _precond(static_settings)
+    o.lin_tol = float(_opt(static_settings, "linear tolerance", 1e-5))
+    o.lin_atol = float(_opt(static_settings, "linear atol", 0.0))
+    o.lin_maxiter = int(_opt(static_settings, "linear maxiter", 0))
+    if reuse_tangent is None:
+        reuse_tangent = all(getattr(getattr(m, "integrand", m), "kernel", 0) in
+                            (N.MODEL_POISSON_SOURCE, N.MODEL_POISSON_VARK, N.MODEL_LINE_TRACTION)
+                            for m in static_settings["model"])
+    o.reuse_tangent = 1 if reuse_tangent else 0
+    return o
+
+
+def solver(dofs, settings, static_settings, **kwargs):
+    _check_backend(static_settings)
+    solver_type = static_settings["solver type"]
+    match solver_type:
+        case "newton":
+            return solve_newton(dofs, settings, static_settings, **kwargs)
+        case "damped newton":
+            return solve_damped_newton(dofs, settings, static_settings, **kwargs)
+        case "linear":
+            return solve_linear(dofs, settings, static_settings, **kwargs), None
+        case _:
+            raise NotImplementedError(f"solver type {solver_type!r} is not available in the 'hip' backend")
+
+
+def solve_newton(dofs, settings, static_settings, newton_tol=1e-8, maxiter=30, **kwargs):
+    """solver.py:740-768 (damping 1.0)."""
+    return solve_damped_newton(dofs, settings, static_settings, newton_tol, 1.0, maxiter)
+
+
+def solve_damped_newton(dofs, settings, static_settings, newton_tol=1e-8, damping_coefficient=0.8, maxiter=30,
+                        **kwargs):
+    """solver.py:770-948 on the device."""
+    _check_backend(static_settings)
+    if "nodal imposition" not in static_settings["solution structure"]:
+        raise NotImplementedError("the 'hip' backend imposes Dirichlet conditions nodally ('nodal imposition')")
+    prob = get_problem(dofs, settings, static_settings)
+    opts = newton_options(static_settings, newton_tol, maxiter, damping_coefficient)
+    u = prob.to_device(prob.flat(dofs))
+    dv = prob.to_device(np.ascontiguousarray(utility.dict_flatten(settings["dirichlet conditions"]),
+                                             dtype=np.float64))
+    st = N.NewtonStats()
+    N.call("apx_newton", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(dv.data_ptr()), C.byref(opts),
+           C.byref(st), current_stream())
+    sol = utility.reshape_as(u.cpu().numpy(), dofs)
+    n = st.iterations
+    details = {
+        "krylov_iterations": st.krylov_iterations, "initial_residual": st.initial_residual,
+        "seconds_total": st.seconds_total, "seconds_residual": st.seconds_residual,
+        "seconds_tangent": st.seconds_tangent, "seconds_krylov": st.seconds_krylov,
+        "fallbacks": st.fallbacks, "history": list(st.history[:min(n, 64)]),
+    }
+    verbose = _opt(static_settings, "verbose", 0)
+    if verbose > 0:
+        for i, r in enumerate(details["history"]):
+            print(f"Residual after Newton iteration {i + 1}: {r}")
+    if st.diverged and n > maxiter:
+        print("Warning: Newton scheme could not converge!")
+    return sol, NewtonInfos(n, st.residual_norm, bool(st.diverged), details)
+
+
+def solve_linear(dofs, settings, static_settings, **kwargs):
+    """solver.py:544-659: one linear solve at dofs; returns Dirichlet values at
+    Dirichlet DOFs and the increment at free DOFs (P5)."""
+    import torch
+    _check_backend(static_settings)
+    prob = get_problem(dofs, settings, static_settings)
+    opts = newton_options(static_settings)
+    dmask = utility.dict_flatten(settings["dirichlet dofs"]).astype(bool)
+    dvals = np.asarray(utility.dict_flatten(settings["dirichlet conditions"]), dtype=np.float64)
+    flat = prob.flat(dofs).copy()
+    flat[dmask] = dvals[dmask]
+    u = prob.to_device(flat)
+    stream = current_stream()
+    r = u.new_empty(prob.n_dofs)
+    N.call("apx_assemble_residual", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(r.data_ptr()), stream)
+    fp64 = opts.precision == N.APX_FP64
+    dt = torch.float64 if fp64 else torch.float32
+    vals = torch.empty(prob.nnz, dtype=dt, device=u.device)
+    N.call("apx_assemble_tangent", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(vals.data_ptr()),
+           opts.precision, stream)
+    r2f = torch.from_numpy(prob.free_dofs().astype(np.int64)).to(u.device)
+    b = (-r[r2f]).to(dt).contiguous()
+    x = torch.empty_like(b)
+    ks = N.KrylovStats()
+    N.call("apx_pcg", prob.plan, C.c_void_p(vals.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+           opts.precision, opts.precond, opts.lin_tol, opts.lin_atol, opts.lin_maxiter, C.byref(ks), stream)
+    out = u.clone()
+    out[r2f] = x.to(torch.float64)
+    return utility.reshape_as(out.cpu().numpy(), dofs)

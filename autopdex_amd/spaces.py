@@ -1,0 +1,45 @@
+"""Solution spaces (host), as autopdex.spaces for the Q1 / Hex8 branches.
+
+fem_iso_line_quad_brick(x, xI, fI, settings, overwrite_diff, n_dim) evaluates
+sum_i N_i(x) fI_i with the Q1 (spaces.py:1913-1922) or Hex8
+(:12057-12081) shape functions.  Passing this function to
+models.mixed_reference_domain_potential selects the isoparametric Q1 / Hex8
+element kernels of the device library; the physical gradient those kernels
+use is (df/dxi) . (dX/dxi)^-1 as in the reference's custom JVP
+(:12162-12191).
+"""
+import numpy as np
+
+
+def _q1(xi):
+    x0 = 0.5 * xi[0]
+    x1 = 0.5 - x0
+    x2 = 0.5 * xi[1]
+    x3 = 0.5 - x2
+    x4 = x0 + 0.5
+    x5 = x2 + 0.5
+    return np.array([x1 * x3, x3 * x4, x4 * x5, x1 * x5])
+
+
+_HEX = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                 [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+
+
+def _hex8(xi):
+    f = 1.0 + _HEX * np.asarray(xi, dtype=np.float64)[None, :]
+    return f[:, 0] * f[:, 1] * f[:, 2] / 8.0
+
+
+def shape_functions(n_nodes, n_dim, xi):
+    if n_dim == 2 and n_nodes == 4:
+        return _q1(xi)
+    if n_dim == 3 and n_nodes == 8:
+        return _hex8(xi)
+    raise NotImplementedError(f"{n_nodes}-node elements in {n_dim}D are not supported by autopdex_amd")
+
+
+def fem_iso_line_quad_brick(x, xI, fI, settings=None, overwrite_diff=False, n_dim=None):
+    xI = np.asarray(xI)
+    n_dim = xI.shape[-1] if n_dim is None else n_dim
+    N = shape_functions(xI.shape[0], n_dim, x)
+    return np.einsum("i,i...->...", N, np.asarray(fI))

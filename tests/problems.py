@@ -1,0 +1,126 @@
+"""Problem builders shared by the tests: each returns the reference-style
+(dofs, settings, static_settings) for the autopdex_amd 'hip' backend and the
+matching oracle description (coords, sets, mask, values, d)."""
+import numpy as np
+
+from autopdex_amd import geometry, mesher, models, seeder, spaces, utility
+from oracle import elements as OE
+from oracle import mesh as OM
+
+UNIT_SQUARE = [[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]]
+TAPERED = [[0.0, 0.0], [48.0, 44.0], [48.0, 60.0], [0.0, 44.0]]
+UNIT_CUBE = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
+
+
+def static(model_list, **extra):
+    s = {
+        "assembling mode": tuple("user potential" for _ in model_list),
+        "solution structure": ("nodal imposition",),
+        "model": tuple(model_list),
+        "solver type": "newton",
+        "solver backend": "hip",
+        "solver": "cg",
+        "type of preconditioner": "jacobi",
+        "verbose": -1,
+    }
+    s.update(extra)
+    return s
+
+
+def poisson_source(n, dict_dofs=True):
+    """Config-1/2 family: unit square, n x n Q1, psdf-polygon Dirichlet mask (corners free)."""
+    coords, elems = mesher.structured_mesh((n, n), UNIT_SQUARE, "quad")
+    mask = geometry.in_sdfs(coords, lambda x: geometry.psdf_polygon(x, UNIT_SQUARE))
+    pot = models.mixed_reference_domain_potential(
+        models.poisson_source_integrand(), {"phi": spaces.fem_iso_line_quad_brick},
+        *seeder.gauss_legendre_nd(dimension=2, order=2), "phi")
+    if dict_dofs:
+        settings = {"connectivity": ({"phi": elems},), "dirichlet dofs": {"phi": mask},
+                    "node coordinates": {"phi": coords},
+                    "dirichlet conditions": utility.dict_zeros_like({"phi": mask})}
+        dofs = utility.dict_zeros_like({"phi": mask})
+    else:
+        settings = {"connectivity": (elems,), "dirichlet dofs": mask, "node coordinates": coords,
+                    "dirichlet conditions": np.zeros(mask.shape)}
+        dofs = np.zeros(mask.shape)
+    oracle = dict(coords=coords, sets=[(OE.PoissonSource(), elems, None)], mask=mask.ravel(),
+                  values=np.zeros(mask.size), d=1)
+    return dofs, settings, static([pot]), oracle
+
+
+def grf(n_nodes_1d, seed, length=0.05, sigma=1.0):
+    """Squared-exponential Gaussian random field on an (m x m) node grid of [0,1]^2
+    by circulant embedding (periodic FFT on a doubled grid), seeded PCG64."""
+    m = n_nodes_1d
+    h = 1.0 / (m - 1)
+    M2 = 2 * m
+    k = np.minimum(np.arange(M2), M2 - np.arange(M2)) * h
+    cov = sigma ** 2 * np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / (2 * length ** 2))
+    lam = np.maximum(np.fft.fft2(cov).real, 0.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xi = rng.standard_normal((M2, M2)) + 1j * rng.standard_normal((M2, M2))
+    f = np.fft.fft2(np.sqrt(lam / (M2 * M2)) * xi)
+    return f.real[:m, :m].ravel()
+
+
+def poisson_vark(n, seed=0):
+    """Config-4 family: -div(k grad u) = f, k = exp(g), g GRF, f ~ N(0,1) per node."""
+    coords, elems = mesher.structured_mesh((n, n), UNIT_SQUARE, "quad")
+    mask = geometry.in_sdfs(coords, lambda x: geometry.psdf_polygon(x, UNIT_SQUARE))
+    g = grf(n + 1, seed)
+    f = np.random.Generator(np.random.PCG64(seed + 1_000_000)).standard_normal(coords.shape[0])
+    pot = models.mixed_reference_domain_potential(
+        models.poisson_variable_coefficient_integrand("log coefficient", "source"),
+        {"u": spaces.fem_iso_line_quad_brick}, *seeder.gauss_legendre_nd(2, 2), "u")
+    settings = {"connectivity": (elems,), "dirichlet dofs": mask, "node coordinates": coords,
+                "dirichlet conditions": np.zeros(mask.shape), "log coefficient": g, "source": f}
+    oracle = dict(coords=coords, sets=[(OE.PoissonVarK(), elems, (g, f))], mask=mask.ravel(),
+                  values=np.zeros(mask.size), d=1)
+    return np.zeros(mask.shape), settings, static([pot]), oracle
+
+
+def neo_hooke_cube(n, lam=10.0, mu=1.0, disp=0.1):
+    """Config-3 family: unit cube n^3 Hex8, x=0 clamped, x=1 u_x = disp (y, z free)."""
+    coords, elems = mesher.structured_mesh((n, n, n), UNIT_CUBE, "brick")
+    left = geometry.in_planes(coords, [0, 0, 0], [1, 0, 0])
+    right = geometry.in_planes(coords, [1, 0, 0], [1, 0, 0])
+    mask = utility.dof_select(left, [True, True, True]) | utility.dof_select(right, [True, False, False])
+    values = np.zeros(mask.shape)
+    values[right, 0] = disp
+    pot = models.mixed_reference_domain_potential(
+        models.hyperelastic_integrand(models.neo_hooke, lam, mu), {"u": spaces.fem_iso_line_quad_brick},
+        *seeder.gauss_legendre_nd(3, 2), "u")
+    settings = {"connectivity": (elems,), "dirichlet dofs": mask, "node coordinates": coords,
+                "dirichlet conditions": values}
+    oracle = dict(coords=coords, sets=[(OE.NeoHooke(lam, mu), elems, None)], mask=mask.ravel(),
+                  values=values.ravel(), d=3)
+    return np.zeros(mask.shape), settings, static([pot]), oracle
+
+
+def tapered_panel(n, lam=40.0, mu=0.4, total_traction=1.0, multiplier=1.0):
+    """Config-5 family: tapered panel n x n Q1, x=0 clamped, vertical traction on x=48."""
+    coords, elems = mesher.structured_mesh((n, n), TAPERED, "quad")
+    left = geometry.in_planes(coords, [0, 0], [1, 0])
+    mask = utility.dof_select(left, [True, True])
+    lines = mesher.boundary_lines((n, n), "right")
+    t = np.array([0.0, total_traction / 16.0])  # right edge length 16
+    pot = models.mixed_reference_domain_potential(
+        models.hyperelastic_integrand(models.neo_hooke, lam, mu), {"u": spaces.fem_iso_line_quad_brick},
+        *seeder.gauss_legendre_nd(2, 2), "u")
+    trac = models.line_traction_potential(t, "load multiplier")
+    settings = {"connectivity": (elems, lines), "dirichlet dofs": mask, "node coordinates": coords,
+                "dirichlet conditions": np.zeros(mask.shape), "load multiplier": multiplier}
+    oracle = dict(coords=coords, sets=[(OE.NeoHooke(lam, mu), elems, None), (OE.LineTraction(t * multiplier), lines, None)],
+                  mask=mask.ravel(), values=np.zeros(mask.size), d=2)
+    return np.zeros(mask.shape), settings, static([pot, trac]), oracle
+
+
+def oracle_newton(o, tol=1e-8, maxiter=30, kind="lapack", lin_tol=1e-10, u0=None):
+    from oracle import assembly as OA
+    from oracle import newton as ON
+    coords, sets, mask, d = o["coords"], o["sets"], o["mask"], o["d"]
+    free = ~mask
+    u0 = np.zeros(mask.size) if u0 is None else u0
+    return ON.newton(u0, lambda u: OA.residual(sets, coords, u, d),
+                     lambda u: OA.reduced_tangent(sets, coords, u, d, free), mask, o["values"],
+                     lambda A, b: ON.linear_solve(A, b, kind, tol=lin_tol)[0], tol=tol, maxiter=maxiter)

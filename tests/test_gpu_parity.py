@@ -1,0 +1,177 @@
+"""GPU parity: HIP kernels (through the C ABI) against the FP64 oracle.
+
+Tolerances: element residuals / tangents / diagonals are FP64 on both sides
+and differ only by summation order -> max-abs error <= 1e-12 of the max-abs
+value; the reduced CSR pattern is compared bit for bit; solutions reach the
+reference's FP64-mode bar (relative solution error 1e-8, relative residual
+1e-10).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(builder, *a, **k):
+    from autopdex_amd import backend
+    dofs, settings, static, o = builder(*a, **k)
+    prob = backend.get_problem(dofs, settings, static)
+    return dofs, settings, static, o, prob
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+CASES = [
+    ("poisson_q1", P.poisson_source, (12,)),
+    ("poisson_vark_q1", P.poisson_vark, (12, 3)),
+    ("neo_hooke_hex8", P.neo_hooke_cube, (4,)),
+    ("neo_hooke_plane_strain_traction", P.tapered_panel, (8,)),
+]
+
+
+@pytest.mark.parametrize("name,builder,args", CASES, ids=[c[0] for c in CASES])
+def test_element_assembly_matches_oracle(torch_gpu, name, builder, args):
+    from autopdex_amd import _native as N
+    from autopdex_amd.backend import current_stream
+    from oracle import assembly as OA
+    torch = torch_gpu
+    print(f"[case] {name} {args}", flush=True)
+    dofs, settings, static, o, prob = _setup(builder, *args)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal(prob.n_dofs) * (0.02 if o["d"] > 1 else 1.0)
+    ud = torch.from_numpy(u).cuda()
+    st = current_stream()
+    # residual
+    r = torch.empty_like(ud)
+    N.call("apx_assemble_residual", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(r.data_ptr()), st)
+    r_ref = OA.residual(o["sets"], o["coords"], u, o["d"])
+    assert _rel(r.cpu().numpy(), r_ref) < 1e-12
+    # reduced CSR: pattern bit-exact, values FP64 and FP32
+    A_ref = OA.reduced_tangent(o["sets"], o["coords"], u, o["d"], ~o["mask"])
+    indptr, indices = prob.csr_pattern()
+    assert np.array_equal(indptr, A_ref.indptr) and np.array_equal(indices, A_ref.indices)
+    v64 = torch.empty(prob.nnz, dtype=torch.float64, device="cuda")
+    N.call("apx_assemble_tangent", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(v64.data_ptr()), N.APX_FP64, st)
+    assert _rel(v64.cpu().numpy(), A_ref.data) < 1e-12
+    v32 = torch.empty(prob.nnz, dtype=torch.float32, device="cuda")
+    N.call("apx_assemble_tangent", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(v32.data_ptr()), N.APX_FP32, st)
+    assert _rel(v32.cpu().numpy().astype(np.float64), A_ref.data) < 1e-5
+    # diagonal (full length)
+    g = torch.empty_like(ud)
+    N.call("apx_assemble_diagonal", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(g.data_ptr()), st)
+    g_ref = OA.diagonal(o["sets"], o["coords"], u, o["d"])
+    assert _rel(g.cpu().numpy(), g_ref) < 1e-12
+
+
+def test_assembly_is_bitwise_deterministic(torch_gpu):
+    from autopdex_amd import _native as N
+    from autopdex_amd.backend import current_stream
+    torch = torch_gpu
+    dofs, settings, static, o, prob = _setup(P.poisson_source, 64)
+    u = torch.from_numpy(np.random.default_rng(1).standard_normal(prob.n_dofs)).cuda()
+    st = current_stream()
+    outs = []
+    for _ in range(3):
+        r = torch.empty_like(u)
+        v = torch.empty(prob.nnz, dtype=torch.float64, device="cuda")
+        N.call("apx_assemble_residual", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(r.data_ptr()), st)
+        N.call("apx_assemble_tangent", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), N.APX_FP64, st)
+        outs.append((r.cpu().numpy().tobytes(), v.cpu().numpy().tobytes()))
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_spmv_and_pcg_match_oracle(torch_gpu):
+    from autopdex_amd import _native as N
+    from autopdex_amd.backend import current_stream
+    from oracle import assembly as OA
+    from oracle import newton as ON
+    torch = torch_gpu
+    dofs, settings, static, o, prob = _setup(P.poisson_source, 40)
+    u = np.zeros(prob.n_dofs)
+    A = OA.reduced_tangent(o["sets"], o["coords"], u, 1, ~o["mask"])
+    st = current_stream()
+    vals = torch.from_numpy(A.data.copy()).cuda()
+    x = np.random.default_rng(3).standard_normal(prob.n_free)
+    xd = torch.from_numpy(x).cuda()
+    y = torch.empty_like(xd)
+    N.call("apx_spmv", prob.plan, C.c_void_p(vals.data_ptr()), C.c_void_p(xd.data_ptr()), C.c_void_p(y.data_ptr()),
+           N.APX_FP64, st)
+    assert _rel(y.cpu().numpy(), A @ x) < 1e-13
+    b = A @ x
+    bd = torch.from_numpy(b).cuda()
+    sol = torch.empty_like(bd)
+    ks = N.KrylovStats()
+    N.call("apx_pcg", prob.plan, C.c_void_p(vals.data_ptr()), C.c_void_p(bd.data_ptr()), C.c_void_p(sol.data_ptr()),
+           N.APX_FP64, N.APX_PRECOND_JACOBI, 1e-10, 0.0, 0, C.byref(ks), st)
+    d = A.diagonal()
+    x_ref, k_ref = ON.pcg(A, b, lambda v: v / d, tol=1e-10)
+    assert ks.converged == 1
+    assert abs(ks.iterations - k_ref) <= max(2, int(0.02 * k_ref))
+    assert np.linalg.norm(sol.cpu().numpy() - x) / np.linalg.norm(x) < 1e-8
+    assert np.linalg.norm(b - A @ sol.cpu().numpy()) / np.linalg.norm(b) < 1.5e-10
+
+
+def test_reference_known_answer_5x5(torch_gpu):
+    """tests/test_dicts_as_dofs_user_potential.py:63 through solver.solver('hip')."""
+    from autopdex_amd import solver
+    dofs, settings, static, o = P.poisson_source(5)
+    sol, infos = solver.solver(dofs, settings, static)
+    assert np.isclose(sol["phi"].sum(), 1.9066412530282952, rtol=1e-10, atol=0)
+    assert infos[2] is False and infos[1] <= 1e-8
+
+
+def test_config1_200x200_matches_superlu(torch_gpu):
+    from autopdex_amd import solver
+    dofs, settings, static, o = P.poisson_source(200)
+    static = dict(static, **{"linear tolerance": 1e-12, "newton tolerance mode": "relative"})
+    sol, infos = solver.solver(dofs, settings, static, newton_tol=1e-10)
+    ref, _ = P.oracle_newton(o)
+    u = sol["phi"].ravel()
+    assert np.linalg.norm(u - ref) / np.linalg.norm(ref) < 1e-8
+    assert infos[1] / infos.details["initial_residual"] <= 1e-10
+
+
+def test_mixed_precision_refinement_reaches_fp64(torch_gpu):
+    from autopdex_amd import solver
+    dofs, settings, static, o = P.poisson_source(128, dict_dofs=False)
+    s64 = dict(static, **{"linear tolerance": 1e-12, "newton tolerance mode": "relative"})
+    s32 = dict(static, **{"linear tolerance": 1e-4, "newton tolerance mode": "relative", "hip precision": "mixed"})
+    u64, i64 = solver.solver(dofs, settings, s64, newton_tol=1e-10)
+    u32, i32 = solver.solver(dofs, settings, s32, newton_tol=1e-10)
+    assert i32[2] is False
+    assert i32[1] / i32.details["initial_residual"] <= 1e-10
+    assert np.linalg.norm(u32 - u64) / np.linalg.norm(u64) < 1e-8
+
+
+def test_neo_hooke_cube_load_steps_match_oracle(torch_gpu):
+    from autopdex_amd import solver
+    n, steps = 4, 3
+    u_dev = None
+    u_ref = None
+    for k in range(1, steps + 1):
+        dofs, settings, static, o = P.neo_hooke_cube(n, disp=0.1 * k / steps)
+        static = dict(static, **{"linear tolerance": 1e-12})
+        if u_dev is None:
+            u_dev, u_ref = dofs, np.zeros(o["mask"].size)
+        sol, infos = solver.solver(u_dev, settings, static, newton_tol=1e-9)
+        ref, rinfo = P.oracle_newton(o, tol=1e-9, u0=u_ref)
+        assert infos[2] is False and rinfo[2] is False
+        assert abs(infos[0] - rinfo[0]) <= 1
+        assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
+        u_dev, u_ref = sol, ref
+
+
+def test_tapered_panel_traction_matches_oracle(torch_gpu):
+    from autopdex_amd import solver
+    dofs, settings, static, o = P.tapered_panel(8, multiplier=1.0)
+    static = dict(static, **{"linear tolerance": 1e-12})
+    sol, infos = solver.solver(dofs, settings, static, newton_tol=1e-9)
+    ref, rinfo = P.oracle_newton(o, tol=1e-9)
+    assert infos[2] is False and rinfo[2] is False
+    assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
