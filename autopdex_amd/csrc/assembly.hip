@@ -157,17 +157,32 @@ static void dispatch(Plan& P, ElemSet& S, const double* u, double* r, void* vals
   }
 }
 
+// Algorithmic (compulsory) bytes of one assembly: coordinates, DOFs and
+// connectivity read once, outputs written once.
+int64_t residual_bytes(const Plan& P) {
+  int64_t b = P.n_nodes * P.dim * 8 + P.n_dofs * 8 * 2;
+  for (auto& s : P.sets) b += s.n_elems * s.npe * 4;
+  return b;
+}
+
+int64_t tangent_bytes(const Plan& P, int precision) {
+  int64_t b = P.n_nodes * P.dim * 8 + P.n_dofs * 8 + P.nnz * (precision == APX_FP64 ? 8 : 4);
+  for (auto& s : P.sets) b += s.n_elems * s.npe * 4;
+  return b;
+}
+
 void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st) {
   fill<double>(r, P.n_dofs, 0.0, st);
   for (auto& S : P.sets) dispatch<double, MODE_RES>(P, S, u, r, nullptr, st);
 }
 
+// Tangent values in the plan's SELL layout (P.sell_total entries, padding 0).
 void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st) {
   if (precision == APX_FP64) {
-    fill<double>((double*)vals, P.nnz, 0.0, st);
+    fill<double>((double*)vals, P.sell_total, 0.0, st);
     for (auto& S : P.sets) dispatch<double, MODE_TAN>(P, S, u, nullptr, vals, st);
   } else {
-    fill<float>((float*)vals, P.nnz, 0.0f, st);
+    fill<float>((float*)vals, P.sell_total, 0.0f, st);
     for (auto& S : P.sets) dispatch<float, MODE_TAN>(P, S, u, nullptr, vals, st);
   }
 }

@@ -10,17 +10,51 @@ void fill(T* p, int64_t n, T v, hipStream_t s);
 void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st);
 void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st);
 void assemble_diagonal(Plan& P, const double* u, double* diag, hipStream_t st);
+int64_t residual_bytes(const Plan& P);
+int64_t tangent_bytes(const Plan& P, int precision);
 
 // krylov.hip
 struct KState {
-  double gamma, pAp, alpha, beta, rr, atol2, bb, rr0;
-  double dot_out;         // scratch result of dot / norm kernels
-  int k, maxiter, done, stagnated;
-  unsigned int counter[4];
+  double gamma, gamma_prev, pAp, alpha, beta, rr, atol2, bb, rr0, rmax2, delta2;
+  double dot_out;  // scratch result of dot / norm kernels
+  int k, maxiter, done, reliable;  // contiguous: copied to the host as one block
 };
 
+void ensure_krylov_scratch(Plan& P, hipStream_t st);
 void spmv(Plan& P, const void* vals, const void* x, void* y, int precision, hipStream_t st);
+void sell_to_csr(Plan& P, const void* sell, void* csr, int precision, hipStream_t st);
+void csr_to_sell(Plan& P, const void* csr, void* sell, int precision, hipStream_t st);
 void jacobi_inverse(Plan& P, const void* vals, void* inv_diag, int precision, hipStream_t st);
+
+// One preconditioned-CG solve on the plan's SELL matrix.  Workspace (r, z, two
+// directions, q, x) comes from the plan.  run() returns when the device sets
+// the stop flag or requests a reliable update (residual drop by delta since the
+// last anchor); reanchor() then installs a new true residual.
+template <class T>
+struct PcgRun {
+  Plan& P;
+  const T* vals;
+  const T* inv;
+  hipStream_t st;
+  int64_t n = 0;
+  T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *xw = nullptr, *x = nullptr;
+  KState* S = nullptr;
+  int maxiter = 0, iterations = 0;
+  int64_t launched = 0;
+  size_t counted = 0;
+  int64_t spmv_bytes = 0, upd_bytes = 0;
+  hipEvent_t evs[2] = {nullptr, nullptr};
+  std::vector<EventPair> ev_spmv, ev_upd;
+
+  PcgRun(Plan& P, const T* vals, const T* inv, hipStream_t st);
+  ~PcgRun();
+  void init(const T* b, T* x, double tol, double atol, int maxiter, double delta);
+  void launch_iteration();
+  KState run();
+  void reanchor(const double* r_full);
+  void collect_timing();
+};
+
 // Jacobi-PCG; workspace taken from the plan.
 apx_krylov_stats pcg(Plan& P, const void* vals, const void* inv_diag, const void* b, void* x, int precision,
                      double tol, double atol, int maxiter, hipStream_t st);

@@ -18,6 +18,16 @@ using namespace apx;
 struct apx_plan : public Plan {};
 
 static inline hipStream_t S(void* s) { return (hipStream_t)s; }
+
+// The plan's internal matrix buffer (SELL layout) for a precision.
+static void* sell_buffer(Plan& P, int precision) {
+  if (precision == APX_FP64) {
+    if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
+    return P.vals64.p;
+  }
+  if (P.vals32.n < (size_t)P.sell_total) P.vals32.alloc(P.sell_total);
+  return P.vals32.p;
+}
 static inline Plan& ready(apx_plan* p) {
   APX_REQUIRE(p != nullptr, APX_ERR_ARG, "null plan");
   APX_REQUIRE(p->finalized, APX_ERR_STATE, "plan not finalized (call apx_plan_finalize)");
@@ -147,7 +157,7 @@ int apx_assemble_residual(apx_plan* plan, const double* u_dev, double* r_dev, vo
   return guarded([&] {
     Plan& P = ready(plan);
     APX_REQUIRE(u_dev && r_dev, APX_ERR_ARG, "null device pointer");
-    ScopedTimer t(P, "residual", S(stream), 0);
+    ScopedTimer t(P, "residual_call", S(stream), residual_bytes(P));
     assemble_residual(P, u_dev, r_dev, S(stream));
   });
 }
@@ -157,8 +167,11 @@ int apx_assemble_tangent(apx_plan* plan, const double* u_dev, void* vals_dev, in
     Plan& P = ready(plan);
     APX_REQUIRE(u_dev && vals_dev, APX_ERR_ARG, "null device pointer");
     APX_REQUIRE(precision == APX_FP64 || precision == APX_FP32, APX_ERR_ARG, "precision must be FP64 or FP32");
-    ScopedTimer t(P, "tangent", S(stream), 0);
-    assemble_tangent(P, u_dev, vals_dev, precision, S(stream));
+    ScopedTimer t(P, precision == APX_FP64 ? "tangent_call_f64" : "tangent_call_f32", S(stream),
+                  tangent_bytes(P, precision));
+    void* sell = sell_buffer(P, precision);
+    assemble_tangent(P, u_dev, sell, precision, S(stream));
+    sell_to_csr(P, sell, vals_dev, precision, S(stream));
   });
 }
 
@@ -176,8 +189,10 @@ int apx_spmv(apx_plan* plan, const void* vals_dev, const void* x_dev, void* y_de
     APX_REQUIRE(vals_dev && x_dev && y_dev, APX_ERR_ARG, "null device pointer");
     APX_REQUIRE(precision == APX_FP64 || precision == APX_FP32, APX_ERR_ARG, "precision must be FP64 or FP32");
     const int64_t ts = precision == APX_FP64 ? 8 : 4;
-    ScopedTimer t(P, "spmv", S(stream), P.nnz * (ts + 4) + (P.n_free + 1) * 4 + P.n_free * 2 * ts);
-    spmv(P, vals_dev, x_dev, y_dev, precision, S(stream));
+    void* sell = sell_buffer(P, precision);
+    csr_to_sell(P, vals_dev, sell, precision, S(stream));
+    ScopedTimer t(P, "spmv", S(stream), P.sell_total * (ts + 4) + (P.n_chunks + 1) * 4 + P.n_free * 2 * ts);
+    spmv(P, sell, x_dev, y_dev, precision, S(stream));
   });
 }
 
@@ -190,15 +205,17 @@ int apx_pcg(apx_plan* plan, const void* vals_dev, const void* b_dev, void* x_dev
     APX_REQUIRE(precond == APX_PRECOND_JACOBI || precond == APX_PRECOND_NONE, APX_ERR_UNSUPPORTED,
                 "apx_pcg supports Jacobi or no preconditioner");
     const size_t ts = precision == APX_FP64 ? 8 : 4;
+    void* sell = sell_buffer(P, precision);
+    csr_to_sell(P, vals_dev, sell, precision, S(stream));
     DevBuf<char> inv;
     inv.alloc(P.n_free * ts);
     if (precond == APX_PRECOND_JACOBI)
-      jacobi_inverse(P, vals_dev, inv.p, precision, S(stream));
+      jacobi_inverse(P, sell, inv.p, precision, S(stream));
     else if (precision == APX_FP64)
       fill<double>((double*)inv.p, P.n_free, 1.0, S(stream));
     else
       fill<float>((float*)inv.p, P.n_free, 1.0f, S(stream));
-    apx_krylov_stats ks = pcg(P, vals_dev, inv.p, b_dev, x_dev, precision, tol, atol, maxiter, S(stream));
+    apx_krylov_stats ks = pcg(P, sell, inv.p, b_dev, x_dev, precision, tol, atol, maxiter, S(stream));
     APX_HIP(hipStreamSynchronize(S(stream)));
     if (stats) *stats = ks;
   });

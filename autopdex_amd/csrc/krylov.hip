@@ -1,19 +1,27 @@
-// Krylov kernels on the reduced CSR system: SpMV, Jacobi-PCG with fused
-// vector updates, deterministic reductions.
+// Krylov kernels on the reduced system in SELL-64 layout: SpMV, Jacobi-PCG
+// with fused vector updates and deterministic reductions, and the
+// reliable-update hook used by mixed-precision CG.
 //
 // Replaces the matrix-free jvp + jax.scipy.sparse.linalg.cg of
 // linear_solve_jax (solver.py:1044-1116) and the host spsolve of
 // linear_solve_scipy (:1493-1539) for the 'hip' backend.
 //
 // One PCG iteration = two kernels:
-//   k_pcg_spmv : p_new = z + beta p_old (fused into the row gather), q = A p_new,
-//                block partials of p.q; the last block forms alpha = gamma / p.q
-//   k_pcg_update: x += alpha p, r -= alpha q, z = D^-1 r, partials of r.z and r.r;
-//                the last block forms beta, the new gamma and the stop flag.
-// Scalars live on the device; kernels exit at entry once the stop flag is set,
-// so the host enqueues iterations in batches and reads the flag one batch late.
-// Reductions: fixed grid, per-block shuffle tree, last block sums the
+//   k_pcg_spmv  : p = z + beta p_old fused into the gather, q = A p, block
+//                 partials of p.q; the last block forms alpha = gamma / p.q.
+//   k_pcg_update: x += alpha p, r -= alpha q, z = D^-1 r, partials of r.z and
+//                 r.r; the last block forms beta, gamma, the stop flag and the
+//                 reliable-update request.
+// Scalars live on the device.  Kernels exit at entry once a flag is set, so
+// the host enqueues iterations in batches and reads the flags one batch late.
+// The two direction buffers alternate by the device iteration counter.
+// Reductions: fixed grid, per-block shuffle tree, the last block sums the
 // partials in index order -> bitwise reproducible.
+//
+// Bandwidth: every thread issues kUnroll independent element loads per pass
+// (vector kernels) and one wavefront owns one 64-row SELL chunk with its
+// value / column loads unrolled by 4, so enough requests are in flight to
+// cover HBM latency; grids are sized to about one pass per thread.
 #include <chrono>
 #include <cstring>
 
@@ -21,8 +29,10 @@
 
 namespace apx {
 
-constexpr int kBS = 256;
-constexpr int kMaxBlocks = 1024;
+constexpr int kBS = 256;        // vector kernels
+constexpr int kSpmvBS = 512;    // SpMV kernels: 8 wavefronts = 8 chunks per block
+constexpr int kUnroll = 4;
+constexpr int kMaxBlocks = 8192;
 
 ScopedTimer::ScopedTimer(Plan& p, const char* n, hipStream_t s, int64_t b) : P(p), name(n), st(s), bytes(b) {
   if (P.timing) (void)hipEventRecord(P.ev0, st);
@@ -39,89 +49,132 @@ ScopedTimer::~ScopedTimer() {
   t.bytes = bytes;
 }
 
-static inline int grid_for(int64_t work_items, int bs = kBS) {
-  int64_t b = (work_items + bs - 1) / bs;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kMaxBlocks));
+static inline int grid_for(int64_t n) {
+  const int64_t per_block = (int64_t)kBS * kUnroll;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + per_block - 1) / per_block, kMaxBlocks));
+}
+static inline int grid_chunks(int64_t n_chunks) {
+  constexpr int wpb = kSpmvBS / 64;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n_chunks + wpb - 1) / wpb, kMaxBlocks));
 }
 
-// Last-block-done reduction of per-block partials (NV values per block).
-// Returns true in every thread of the last block; sums[] then holds totals.
-template <int NV>
-__device__ inline bool last_block_reduce(double (&v)[NV], double* partials, unsigned int* counter,
-                                         double (&sums)[NV]) {
-  __shared__ double lds[kBS / 64];
-  __shared__ bool is_last;
-  double bs[NV];
-#pragma unroll
-  for (int j = 0; j < NV; ++j) bs[j] = block_sum<kBS>(v[j], lds);
-  // Partials cross XCDs (separate L2s): agent-scope atomics + fences make the
-  // stores of every block visible to the last one.
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-      __hip_atomic_store(&partials[j * kMaxBlocks + blockIdx.x], bs[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == gridDim.x - 1);
+void ensure_krylov_scratch(Plan& P, hipStream_t st) {
+  if (P.partials.n < (size_t)(4 * kMaxBlocks)) P.partials.alloc(4 * kMaxBlocks);
+  if (P.kstate.n < 64) {
+    P.kstate.alloc(64);
+    APX_HIP(hipMemsetAsync(P.kstate.p, 0, P.kstate.bytes(), st));
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// Cross-block reductions are split in two launches on the same stream: every
+// block writes its partial sums (plain stores), then a one-block finalize
+// kernel adds them in index order and updates the device scalars.  The kernel
+// boundary makes the partials visible across the XCDs' separate L2 caches; an
+// in-kernel last-block scheme would need an agent-scope release per block,
+// which writes back the whole L2 each time.
+template <int BS, int NV>
+__device__ inline void block_partials(double (&v)[NV], double* partials) {
+  __shared__ double lds[BS / 64];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const double s = block_sum<BS>(v[j], lds);
+    if (threadIdx.x == 0) partials[j * kMaxBlocks + blockIdx.x] = s;
+  }
+}
+
+constexpr int kFinBS = 1024;
+
+// Sum NV rows of nb partials (fixed order) -> valid in all threads.
+template <int NV>
+__device__ inline void finalize_sums(const double* partials, int nb, double (&sums)[NV]) {
+  __shared__ double lds[kFinBS / 64];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBS)
-      s += __hip_atomic_load(&partials[j * kMaxBlocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sums[j] = block_sum<kBS>(s, lds);
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
-
-// ---- SpMV ------------------------------------------------------------------------
-template <class T, int G>
-__global__ __launch_bounds__(kBS) void k_spmv(int64_t n, const int32_t* __restrict__ indptr,
-                                              const int32_t* __restrict__ indices, const T* __restrict__ vals,
-                                              const T* __restrict__ x, T* __restrict__ y) {
-  const int lane = threadIdx.x % G;
-  for (int64_t row = (blockIdx.x * (int64_t)kBS + threadIdx.x) / G; row < n;
-       row += (int64_t)gridDim.x * (kBS / G)) {
-    const int32_t b = indptr[row], e = indptr[row + 1];
-    T s = 0;
-    for (int32_t k = b + lane; k < e; k += G) s += vals[k] * x[indices[k]];
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-    if (lane == 0) y[row] = s;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < nb; i += kFinBS) s += partials[j * kMaxBlocks + i];
+    sums[j] = block_sum<kFinBS>(s, lds);
   }
 }
 
-static int lanes_per_row(const Plan& P) {
-  const double avg = P.n_free ? (double)P.nnz / (double)P.n_free : 1.0;
-  if (avg <= 12) return 4;
-  if (avg <= 24) return 8;
-  if (avg <= 48) return 16;
-  return 32;
+// Row gather of one SELL chunk: sum_k vals[k] * f(cols[k]) for this lane's row.
+template <class T, class F>
+__device__ __forceinline__ T sell_row(const T* __restrict__ v, const int32_t* __restrict__ ci, int w, F f) {
+  T s = 0;
+  int k = 0;
+  for (; k + 4 <= w; k += 4) {
+    const int32_t c0 = ci[(k + 0) * 64], c1 = ci[(k + 1) * 64], c2 = ci[(k + 2) * 64], c3 = ci[(k + 3) * 64];
+    const T v0 = v[(k + 0) * 64], v1 = v[(k + 1) * 64], v2 = v[(k + 2) * 64], v3 = v[(k + 3) * 64];
+    s += v0 * f(c0);
+    s += v1 * f(c1);
+    s += v2 * f(c2);
+    s += v3 * f(c3);
+  }
+  for (; k < w; ++k) s += v[k * 64] * f(ci[k * 64]);
+  return s;
 }
 
+// ---- SpMV (SELL-64: one wavefront per chunk, lane = row) --------------------------
 template <class T>
-static void spmv_t(Plan& P, const T* vals, const T* x, T* y, hipStream_t st) {
-  const int G = lanes_per_row(P);
-  const int64_t rows_per_block = kBS / G;
-  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((P.n_free + rows_per_block - 1) / rows_per_block, 1 << 20));
-  switch (G) {
-    case 4: hipLaunchKernelGGL((k_spmv<T, 4>), dim3(blocks), dim3(kBS), 0, st, P.n_free, P.indptr.p, P.indices.p, vals, x, y); break;
-    case 8: hipLaunchKernelGGL((k_spmv<T, 8>), dim3(blocks), dim3(kBS), 0, st, P.n_free, P.indptr.p, P.indices.p, vals, x, y); break;
-    case 16: hipLaunchKernelGGL((k_spmv<T, 16>), dim3(blocks), dim3(kBS), 0, st, P.n_free, P.indptr.p, P.indices.p, vals, x, y); break;
-    default: hipLaunchKernelGGL((k_spmv<T, 32>), dim3(blocks), dim3(kBS), 0, st, P.n_free, P.indptr.p, P.indices.p, vals, x, y); break;
+__global__ __launch_bounds__(kSpmvBS) void k_spmv_sell(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
+                                                       const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                       const T* __restrict__ x, T* __restrict__ y) {
+  constexpr int wpb = kSpmvBS / 64;
+  const int lane = threadIdx.x & 63;
+  for (int64_t c = blockIdx.x * (int64_t)wpb + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * wpb) {
+    const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
+    const T s = sell_row<T>(vals + off + lane, cols + off + lane, w, [&](int32_t col) { return x[col]; });
+    const int64_t row = c * 64 + lane;
+    if (row < n) y[row] = s;
   }
-  APX_LAUNCH_CHECK();
 }
 
 void spmv(Plan& P, const void* vals, const void* x, void* y, int precision, hipStream_t st) {
+  const int g = grid_chunks(P.n_chunks);
   if (precision == APX_FP64)
-    spmv_t<double>(P, (const double*)vals, (const double*)x, (double*)y, st);
+    hipLaunchKernelGGL(k_spmv_sell<double>, dim3(g), dim3(kSpmvBS), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p,
+                       P.sell_cols.p, (const double*)vals, (const double*)x, (double*)y);
   else
-    spmv_t<float>(P, (const float*)vals, (const float*)x, (float*)y, st);
+    hipLaunchKernelGGL(k_spmv_sell<float>, dim3(g), dim3(kSpmvBS), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p,
+                       P.sell_cols.p, (const float*)vals, (const float*)x, (float*)y);
+  APX_LAUNCH_CHECK();
+}
+
+// ---- layout conversion --------------------------------------------------------------
+template <class T>
+__global__ void k_sell_to_csr(int64_t nnz, const int32_t* csr2sell, const T* sell, T* csr) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+    csr[k] = sell[csr2sell[k]];
+}
+template <class T>
+__global__ void k_csr_to_sell(int64_t nnz, const int32_t* csr2sell, const T* csr, T* sell) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+    sell[csr2sell[k]] = csr[k];
+}
+
+void sell_to_csr(Plan& P, const void* sell, void* csr, int precision, hipStream_t st) {
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((P.nnz + 255) / 256, 65536));
+  if (precision == APX_FP64)
+    hipLaunchKernelGGL(k_sell_to_csr<double>, dim3(g), dim3(256), 0, st, P.nnz, P.csr2sell.p, (const double*)sell,
+                       (double*)csr);
+  else
+    hipLaunchKernelGGL(k_sell_to_csr<float>, dim3(g), dim3(256), 0, st, P.nnz, P.csr2sell.p, (const float*)sell,
+                       (float*)csr);
+  APX_LAUNCH_CHECK();
+}
+
+void csr_to_sell(Plan& P, const void* csr, void* sell, int precision, hipStream_t st) {
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((P.nnz + 255) / 256, 65536));
+  if (precision == APX_FP64) {
+    fill<double>((double*)sell, P.sell_total, 0.0, st);
+    hipLaunchKernelGGL(k_csr_to_sell<double>, dim3(g), dim3(256), 0, st, P.nnz, P.csr2sell.p, (const double*)csr,
+                       (double*)sell);
+  } else {
+    fill<float>((float*)sell, P.sell_total, 0.0f, st);
+    hipLaunchKernelGGL(k_csr_to_sell<float>, dim3(g), dim3(256), 0, st, P.nnz, P.csr2sell.p, (const float*)csr,
+                       (float*)sell);
+  }
+  APX_LAUNCH_CHECK();
 }
 
 // ---- Jacobi ------------------------------------------------------------------------
@@ -132,212 +185,315 @@ __global__ void k_inv_diag(int64_t n, const int32_t* diag_slot, const T* vals, T
 }
 
 void jacobi_inverse(Plan& P, const void* vals, void* inv, int precision, hipStream_t st) {
-  const int g = (int)std::min<int64_t>((P.n_free + 255) / 256, 65536);
+  const int g = std::max(1, (int)std::min<int64_t>((P.n_free + 255) / 256, 65536));
   if (precision == APX_FP64)
-    hipLaunchKernelGGL(k_inv_diag<double>, dim3(std::max(g, 1)), dim3(256), 0, st, P.n_free, P.diag_slot.p,
-                       (const double*)vals, (double*)inv);
+    hipLaunchKernelGGL(k_inv_diag<double>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const double*)vals,
+                       (double*)inv);
   else
-    hipLaunchKernelGGL(k_inv_diag<float>, dim3(std::max(g, 1)), dim3(256), 0, st, P.n_free, P.diag_slot.p,
-                       (const float*)vals, (float*)inv);
+    hipLaunchKernelGGL(k_inv_diag<float>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const float*)vals,
+                       (float*)inv);
   APX_LAUNCH_CHECK();
 }
 
-// ---- PCG ---------------------------------------------------------------------------
+// ---- PCG kernels --------------------------------------------------------------------
+__device__ __forceinline__ bool halted(const KState* S) { return S->done | S->reliable; }
+
+// Elementwise pass with kUnroll independent elements per thread and pass.
+#define APX_FOR_ELEMENTS(n, BODY_LOAD, BODY_STORE)                                         \
+  for (int64_t base = blockIdx.x * (int64_t)kBS * kUnroll + threadIdx.x; base < (n);        \
+       base += (int64_t)gridDim.x * kBS * kUnroll) {                                        \
+    _Pragma("unroll") for (int j = 0; j < kUnroll; ++j) {                                   \
+      const int64_t i = base + (int64_t)j * kBS;                                            \
+      if (i < (n)) { BODY_LOAD }                                                            \
+    }                                                                                       \
+    _Pragma("unroll") for (int j = 0; j < kUnroll; ++j) {                                   \
+      const int64_t i = base + (int64_t)j * kBS;                                            \
+      if (i < (n)) { BODY_STORE }                                                           \
+    }                                                                                       \
+  }
+
 template <class T>
 __global__ __launch_bounds__(kBS) void k_pcg_init(int64_t n, const T* __restrict__ b, const T* __restrict__ inv,
                                                   T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                  T* __restrict__ p_old, KState* S, double* partials,
-                                                  double tol, double atol, int maxiter) {
+                                                  T* __restrict__ pa, KState* S, double* partials, double tol,
+                                                  double atol, int maxiter, double delta) {
   double v[2] = {0.0, 0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    const T bi = b[i];
-    const T zi = inv[i] * bi;
+  T bv[kUnroll], iv[kUnroll];
+  APX_FOR_ELEMENTS(n, { bv[j] = b[i]; iv[j] = inv[i]; }, {
+    const T zi = iv[j] * bv[j];
     x[i] = 0;
-    r[i] = bi;
+    r[i] = bv[j];
     z[i] = zi;
-    p_old[i] = 0;
-    v[0] += (double)bi * (double)zi;
-    v[1] += (double)bi * (double)bi;
-  }
+    pa[i] = 0;
+    v[0] += (double)bv[j] * (double)zi;
+    v[1] += (double)bv[j] * (double)bv[j];
+  })
+  block_partials<kBS, 2>(v, partials);
+}
+
+__global__ __launch_bounds__(kFinBS) void k_pcg_init_fin(KState* S, const double* partials, int nb, double tol,
+                                                         double atol, int maxiter, double delta) {
   double s[2];
-  if (last_block_reduce<2>(v, partials, &S->counter[0], s) && threadIdx.x == 0) {
+  finalize_sums<2>(partials, nb, s);
+  if (threadIdx.x == 0) {
     S->gamma = s[0];
+    S->gamma_prev = s[0];
     S->rr = s[1];
     S->rr0 = s[1];
     S->bb = s[1];
+    S->rmax2 = s[1];
+    S->delta2 = delta * delta;
     S->atol2 = fmax(tol * tol * s[1], atol * atol);
     S->beta = 0.0;
     S->k = 0;
     S->maxiter = maxiter;
-    S->stagnated = 0;
+    S->reliable = 0;
     S->done = !(s[1] > S->atol2 && 0 < maxiter);
   }
 }
 
-template <class T, int G>
-__global__ __launch_bounds__(kBS) void k_pcg_spmv(int64_t n, const int32_t* __restrict__ indptr,
-                                                  const int32_t* __restrict__ indices, const T* __restrict__ vals,
-                                                  const T* __restrict__ z, const T* __restrict__ p_old,
-                                                  T* __restrict__ p_new, T* __restrict__ q, KState* S,
-                                                  double* partials) {
-  if (S->done) return;
+template <class T>
+__global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
+                                                      const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                      const T* __restrict__ z, T* __restrict__ pa, T* __restrict__ pb,
+                                                      T* __restrict__ q, KState* S, double* partials) {
+  if (halted(S)) return;
+  constexpr int wpb = kSpmvBS / 64;
   const T beta = (T)S->beta;
-  const int lane = threadIdx.x % G;
+  const bool odd = S->k & 1;
+  const T* __restrict__ po = odd ? pb : pa;
+  T* __restrict__ pn = odd ? pa : pb;
+  const int lane = threadIdx.x & 63;
   double v[1] = {0.0};
-  for (int64_t row = (blockIdx.x * (int64_t)kBS + threadIdx.x) / G; row < n;
-       row += (int64_t)gridDim.x * (kBS / G)) {
-    const int32_t b = indptr[row], e = indptr[row + 1];
-    T s = 0;
-    for (int32_t k = b + lane; k < e; k += G) {
-      const int32_t c = indices[k];
-      s += vals[k] * (z[c] + beta * p_old[c]);
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-    if (lane == 0) {
-      const T pi = z[row] + beta * p_old[row];
-      p_new[row] = pi;
+  for (int64_t c = blockIdx.x * (int64_t)wpb + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * wpb) {
+    const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
+    const T s = sell_row<T>(vals + off + lane, cols + off + lane, w,
+                            [&](int32_t col) { return z[col] + beta * po[col]; });
+    const int64_t row = c * 64 + lane;
+    if (row < n) {
+      const T pi = z[row] + beta * po[row];
+      pn[row] = pi;
       q[row] = s;
       v[0] += (double)pi * (double)s;
     }
   }
+  block_partials<kSpmvBS, 1>(v, partials);
+}
+
+__global__ __launch_bounds__(kFinBS) void k_pcg_alpha_fin(KState* S, const double* partials, int nb) {
+  if (halted(S)) return;
   double tot[1];
-  if (last_block_reduce<1>(v, partials, &S->counter[1], tot) && threadIdx.x == 0) {
+  finalize_sums<1>(partials, nb, tot);
+  if (threadIdx.x == 0) {
     S->pAp = tot[0];
     S->alpha = S->gamma / tot[0];
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(kBS) void k_pcg_update(int64_t n, const T* __restrict__ inv, const T* __restrict__ p,
-                                                    const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                                                    T* __restrict__ z, KState* S, double* partials) {
-  if (S->done) return;
+__global__ __launch_bounds__(kBS) void k_pcg_update(int64_t n, const T* __restrict__ inv, const T* __restrict__ pa,
+                                                    const T* __restrict__ pb, const T* __restrict__ q,
+                                                    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                    KState* S, double* partials) {
+  if (halted(S)) return;
   const T alpha = (T)S->alpha;
+  const T* __restrict__ p = (S->k & 1) ? pa : pb;  // written by k_pcg_spmv of this iteration
   double v[2] = {0.0, 0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] - alpha * q[i];
-    const T zi = inv[i] * ri;
+  T pv[kUnroll], qv[kUnroll], xv[kUnroll], rv[kUnroll], iv[kUnroll];
+  APX_FOR_ELEMENTS(n, {
+    pv[j] = p[i];
+    qv[j] = q[i];
+    xv[j] = x[i];
+    rv[j] = r[i];
+    iv[j] = inv[i];
+  }, {
+    x[i] = xv[j] + alpha * pv[j];
+    const T ri = rv[j] - alpha * qv[j];
+    const T zi = iv[j] * ri;
     r[i] = ri;
     z[i] = zi;
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
-  }
+  })
+  block_partials<kBS, 2>(v, partials);
+}
+
+__global__ __launch_bounds__(kFinBS) void k_pcg_beta_fin(KState* S, const double* partials, int nb) {
+  if (halted(S)) return;
   double s[2];
-  if (last_block_reduce<2>(v, partials, &S->counter[2], s) && threadIdx.x == 0) {
+  finalize_sums<2>(partials, nb, s);
+  if (threadIdx.x == 0) {
+    S->gamma_prev = S->gamma;
     S->beta = s[0] / S->gamma;
     S->gamma = s[0];
     S->rr = s[1];
     S->k += 1;
     S->done = !(s[1] > S->atol2 && S->k < S->maxiter);
+    if (!S->done && S->delta2 > 0.0 && s[1] < S->delta2 * S->rmax2) S->reliable = 1;
   }
 }
+
+// Replace the recursive residual by b_new = -r_full[free] (a true residual
+// computed in FP64 by the caller) and zero the correction x; the direction p
+// is kept (reliable update, Sleijpen & van der Vorst 1996).
+template <class T>
+__global__ __launch_bounds__(kBS) void k_pcg_reanchor(int64_t n, const int32_t* __restrict__ red2full,
+                                                      const double* __restrict__ rfull, const T* __restrict__ inv,
+                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                      KState* S, double* partials) {
+  double v[2] = {0.0, 0.0};
+  T rv[kUnroll], iv[kUnroll];
+  APX_FOR_ELEMENTS(n, { rv[j] = (T)(-rfull[red2full[i]]); iv[j] = inv[i]; }, {
+    const T zi = iv[j] * rv[j];
+    x[i] = 0;
+    r[i] = rv[j];
+    z[i] = zi;
+    v[0] += (double)rv[j] * (double)zi;
+    v[1] += (double)rv[j] * (double)rv[j];
+  })
+  block_partials<kBS, 2>(v, partials);
+}
+
+__global__ __launch_bounds__(kFinBS) void k_pcg_reanchor_fin(KState* S, const double* partials, int nb) {
+  double s[2];
+  finalize_sums<2>(partials, nb, s);
+  if (threadIdx.x == 0) {
+    S->beta = s[0] / S->gamma_prev;
+    S->gamma = s[0];
+    S->rr = s[1];
+    S->rmax2 = s[1];
+    S->reliable = 0;
+    S->done = !(s[1] > S->atol2 && S->k < S->maxiter);
+  }
+}
+
+// ---- PCG driver ----------------------------------------------------------------------
+template <class T>
+PcgRun<T>::PcgRun(Plan& P_, const T* vals_, const T* inv_, hipStream_t st_) : P(P_), vals(vals_), inv(inv_), st(st_) {
+  n = P.n_free;
+  T* w;
+  if constexpr (sizeof(T) == 8) {
+    if (P.w64.n < (size_t)(6 * n)) P.w64.alloc(6 * n);
+    w = (T*)P.w64.p;
+  } else {
+    if (P.w32.n < (size_t)(6 * n)) P.w32.alloc(6 * n);
+    w = (T*)P.w32.p;
+  }
+  r = w; z = w + n; pa = w + 2 * n; pb = w + 3 * n; q = w + 4 * n; xw = w + 5 * n;
+  ensure_krylov_scratch(P, st);
+  S = (KState*)P.kstate.p;
+  APX_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+  APX_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+  // SELL values + columns, chunk offsets, z and p_old read, p_new and q written
+  spmv_bytes = P.sell_total * (int64_t)(sizeof(T) + 4) + (P.n_chunks + 1) * 4 + n * 4 * (int64_t)sizeof(T);
+  upd_bytes = n * 8 * (int64_t)sizeof(T);  // reads D^-1, p, q, x, r; writes x, r, z
+}
+
+template <class T>
+PcgRun<T>::~PcgRun() {
+  (void)hipEventDestroy(evs[0]);
+  (void)hipEventDestroy(evs[1]);
+}
+
+template <class T>
+void PcgRun<T>::init(const T* b, T* x_, double tol, double atol, int maxiter_, double delta) {
+  x = x_ ? x_ : xw;
+  maxiter = maxiter_ <= 0 ? (int)std::min<int64_t>(10 * n, INT32_MAX / 2) : maxiter_;
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_pcg_init<T>, dim3(g), dim3(kBS), 0, st, n, b, inv, x, r, z, pa, S, P.partials.p, tol, atol,
+                     maxiter, delta);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, g, tol, atol, maxiter, delta);
+  APX_LAUNCH_CHECK();
+}
+
+template <class T>
+void PcgRun<T>::launch_iteration() {
+  const int gs = grid_chunks(P.n_chunks), gu = grid_for(n);
+  EventPair e1 = timing_begin(P, st);
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals,
+                     z, pa, pb, q, S, P.partials.p);
+  timing_end(P, e1, st);
+  hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gs);
+  EventPair e2 = timing_begin(P, st);
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gu), dim3(kBS), 0, st, n, inv, pa, pb, q, x, r, z, S, P.partials.p);
+  timing_end(P, e2, st);
+  hipLaunchKernelGGL(k_pcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gu);
+  if (P.timing) {
+    ev_spmv.push_back(e1);
+    ev_upd.push_back(e2);
+  }
+}
+
+template <class T>
+KState PcgRun<T>::run() {
+  constexpr int kBatch = 16;
+  volatile int* flag = (volatile int*)P.pinned;  // [slot*4 + {k, maxiter, done, reliable}]
+  auto enqueue_batch = [&](int slot) {
+    for (int i = 0; i < kBatch; ++i) launch_iteration();
+    APX_LAUNCH_CHECK();
+    APX_HIP(hipMemcpyAsync((void*)(flag + 4 * slot), &S->k, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    APX_HIP(hipEventRecord(evs[slot], st));
+    launched += kBatch;
+  };
+  auto halted_host = [&](int slot) { return flag[4 * slot + 2] != 0 || flag[4 * slot + 3] != 0; };
+  // flags after init / reanchor
+  APX_HIP(hipMemcpyAsync((void*)(flag + 8), &S->k, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+  APX_HIP(hipStreamSynchronize(st));
+  if (!halted_host(2)) {
+    const int64_t limit = (int64_t)launched + maxiter + 2 * kBatch;
+    enqueue_batch(0);
+    int slot = 0;
+    for (;;) {
+      enqueue_batch(slot ^ 1);
+      APX_HIP(hipEventSynchronize(evs[slot]));
+      if (halted_host(slot) || launched >= limit) break;
+      slot ^= 1;
+    }
+  }
+  APX_HIP(hipStreamSynchronize(st));
+  KState hs;
+  APX_HIP(hipMemcpy(&hs, S, sizeof(KState), hipMemcpyDeviceToHost));
+  iterations = hs.k;
+  return hs;
+}
+
+template <class T>
+void PcgRun<T>::reanchor(const double* r_full) {
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_pcg_reanchor<T>, dim3(g), dim3(kBS), 0, st, n, P.red2full.p, r_full, inv, x, r, z, S,
+                     P.partials.p);
+  hipLaunchKernelGGL(k_pcg_reanchor_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, g);
+  APX_LAUNCH_CHECK();
+}
+
+template <class T>
+void PcgRun<T>::collect_timing() {
+  if (!P.timing) return;
+  const bool f64 = sizeof(T) == 8;
+  // only launches that executed an iteration are counted; the rest exited at entry
+  const size_t k = (size_t)iterations - counted;
+  timing_collect(P, f64 ? "pcg_spmv_f64" : "pcg_spmv_f32", ev_spmv, spmv_bytes, k);
+  timing_collect(P, f64 ? "pcg_update_f64" : "pcg_update_f32", ev_upd, upd_bytes, k);
+  counted = iterations;
+}
+
+template struct PcgRun<double>;
+template struct PcgRun<float>;
 
 template <class T>
 static apx_krylov_stats pcg_t(Plan& P, const T* vals, const T* inv, const T* b, T* x, double tol, double atol,
                               int maxiter, hipStream_t st) {
   auto t0 = std::chrono::steady_clock::now();
-  const int64_t n = P.n_free;
-  // workspace: r, z, pA, pB, q
-  T* w;
-  if constexpr (sizeof(T) == 8) {
-    if (P.w64.n < (size_t)(5 * n)) P.w64.alloc(5 * n);
-    w = (T*)P.w64.p;
-  } else {
-    if (P.w32.n < (size_t)(5 * n)) P.w32.alloc(5 * n);
-    w = (T*)P.w32.p;
-  }
-  T *r = w, *z = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *q = w + 4 * n;
-  if (P.partials.n < (size_t)(4 * kMaxBlocks)) P.partials.alloc(4 * kMaxBlocks);
-  if (P.kstate.n < 64) {
-    P.kstate.alloc(64);
-    APX_HIP(hipMemsetAsync(P.kstate.p, 0, P.kstate.bytes(), st));
-  }
-  KState* S = (KState*)P.kstate.p;
-  if (maxiter <= 0) maxiter = (int)std::min<int64_t>(10 * n, INT32_MAX / 2);
-
-  const int gv = grid_for(n);
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv), dim3(kBS), 0, st, n, b, inv, x, r, z, pa, S, P.partials.p, tol, atol,
-                     maxiter);
-  APX_LAUNCH_CHECK();
-
-  const int G = lanes_per_row(P);
-  const int gs = grid_for(n * G);
-  const int64_t spmv_bytes = P.nnz * (int64_t)(sizeof(T) + 4) + (n + 1) * 4 + n * 4 * (int64_t)sizeof(T);
-  const int64_t upd_bytes = n * 9 * (int64_t)sizeof(T);
-  constexpr int kBatch = 16;
-  int launched = 0;
-  int polled_batches = 0;
-  // pinned flag slots: one per batch parity
-  volatile double* flag = (volatile double*)P.pinned;
-  auto launch_iter = [&](T* po, T* pn) {
-    switch (G) {
-      case 4: hipLaunchKernelGGL((k_pcg_spmv<T, 4>), dim3(gs), dim3(kBS), 0, st, n, P.indptr.p, P.indices.p, vals, z, po, pn, q, S, P.partials.p); break;
-      case 8: hipLaunchKernelGGL((k_pcg_spmv<T, 8>), dim3(gs), dim3(kBS), 0, st, n, P.indptr.p, P.indices.p, vals, z, po, pn, q, S, P.partials.p); break;
-      case 16: hipLaunchKernelGGL((k_pcg_spmv<T, 16>), dim3(gs), dim3(kBS), 0, st, n, P.indptr.p, P.indices.p, vals, z, po, pn, q, S, P.partials.p); break;
-      default: hipLaunchKernelGGL((k_pcg_spmv<T, 32>), dim3(gs), dim3(kBS), 0, st, n, P.indptr.p, P.indices.p, vals, z, po, pn, q, S, P.partials.p); break;
-    }
-    hipLaunchKernelGGL(k_pcg_update<T>, dim3(gv), dim3(kBS), 0, st, n, inv, pn, q, x, r, z, S, P.partials.p);
-  };
-  hipEvent_t evs[2];
-  APX_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
-  APX_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
-  bool timing = P.timing;
-  if (timing) APX_HIP(hipEventRecord(P.ev0, st));
-  // Double-buffered polling: batch i+1 is queued before the flag of batch i is read.
-  auto enqueue_batch = [&](int slot) {
-    for (int i = 0; i < kBatch; ++i) {
-      T* po = (launched & 1) ? pb : pa;
-      T* pn = (launched & 1) ? pa : pb;
-      launch_iter(po, pn);
-      ++launched;
-    }
-    APX_LAUNCH_CHECK();
-    APX_HIP(hipMemcpyAsync((void*)(flag + 2 * slot), &S->done, sizeof(int), hipMemcpyDeviceToHost, st));
-    APX_HIP(hipEventRecord(evs[slot], st));
-  };
-  // done flag from init
-  int done_host = 0;
-  APX_HIP(hipMemcpyAsync((void*)(flag + 4), &S->done, sizeof(int), hipMemcpyDeviceToHost, st));
-  APX_HIP(hipStreamSynchronize(st));
-  std::memcpy(&done_host, (const void*)(flag + 4), sizeof(int));
-  if (!done_host) {
-    enqueue_batch(0);
-    int slot = 0;
-    for (;;) {
-      if (launched < maxiter) enqueue_batch(slot ^ 1);
-      APX_HIP(hipEventSynchronize(evs[slot]));
-      ++polled_batches;
-      std::memcpy(&done_host, (const void*)(flag + 2 * slot), sizeof(int));
-      if (done_host || launched >= maxiter + kBatch) break;
-      slot ^= 1;
-    }
-  }
-  APX_HIP(hipStreamSynchronize(st));
-  if (timing) {
-    APX_HIP(hipEventRecord(P.ev1, st));
-    APX_HIP(hipEventSynchronize(P.ev1));
-  }
-  (void)hipEventDestroy(evs[0]);
-  (void)hipEventDestroy(evs[1]);
-  // odd iteration count -> the final direction sits in pb, irrelevant for x.
-  KState hs;
-  APX_HIP(hipMemcpy(&hs, S, sizeof(KState), hipMemcpyDeviceToHost));
+  PcgRun<T> run(P, vals, inv, st);
+  run.init(b, x, tol, atol, maxiter, 0.0);
+  KState hs = run.run();
+  run.collect_timing();
   apx_krylov_stats out{};
   out.iterations = hs.k;
   out.converged = hs.rr <= hs.atol2;
   out.initial_residual = sqrt(hs.rr0);
   out.final_residual = sqrt(hs.rr);
   out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (timing) {
-    float ms = 0.f;
-    APX_HIP(hipEventElapsedTime(&ms, P.ev0, P.ev1));
-    auto& t = P.timers["pcg_iteration"];
-    t.total_ms += ms;
-    t.count += std::max(hs.k, 1);
-    t.bytes = spmv_bytes + upd_bytes;
-  }
   return out;
 }
 
@@ -353,27 +509,27 @@ apx_krylov_stats pcg(Plan& P, const void* vals, const void* inv, const void* b, 
 __global__ __launch_bounds__(kBS) void k_free_norm(int64_t n, const int32_t* red2full, const double* r, KState* S,
                                                    double* partials) {
   double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    const double ri = r[red2full[i]];
-    v[0] += ri * ri;
-  }
+  double rv[kUnroll];
+  APX_FOR_ELEMENTS(n, { rv[j] = r[red2full[i]]; }, { v[0] += rv[j] * rv[j]; })
+  block_partials<kBS, 1>(v, partials);
+}
+
+__global__ __launch_bounds__(kFinBS) void k_norm_fin(KState* S, const double* partials, int nb) {
   double s[1];
-  if (last_block_reduce<1>(v, partials, &S->counter[3], s) && threadIdx.x == 0) S->dot_out = sqrt(s[0]);
+  finalize_sums<1>(partials, nb, s);
+  if (threadIdx.x == 0) S->dot_out = sqrt(s[0]);
 }
 
 double free_norm(Plan& P, const double* r_full, hipStream_t st) {
-  if (P.partials.n < (size_t)(4 * kMaxBlocks)) P.partials.alloc(4 * kMaxBlocks);
-  if (P.kstate.n < 64) {
-    P.kstate.alloc(64);
-    APX_HIP(hipMemsetAsync(P.kstate.p, 0, P.kstate.bytes(), st));
-  }
+  ensure_krylov_scratch(P, st);
   KState* S = (KState*)P.kstate.p;
-  hipLaunchKernelGGL(k_free_norm, dim3(grid_for(P.n_free)), dim3(kBS), 0, st, P.n_free, P.red2full.p, r_full, S,
-                     P.partials.p);
+  const int g = grid_for(P.n_free);
+  hipLaunchKernelGGL(k_free_norm, dim3(g), dim3(kBS), 0, st, P.n_free, P.red2full.p, r_full, S, P.partials.p);
+  hipLaunchKernelGGL(k_norm_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, g);
   APX_LAUNCH_CHECK();
-  APX_HIP(hipMemcpyAsync(P.pinned + 8, &S->dot_out, sizeof(double), hipMemcpyDeviceToHost, st));
+  APX_HIP(hipMemcpyAsync(P.pinned + 16, &S->dot_out, sizeof(double), hipMemcpyDeviceToHost, st));
   APX_HIP(hipStreamSynchronize(st));
-  return P.pinned[8];
+  return P.pinned[16];
 }
 
 template <class T>

@@ -12,6 +12,14 @@
 // The residual after an update is reused as the next iteration's right-hand
 // side (the reference assembles the same vector twice).  Only one scalar
 // (the residual norm) comes back to the host per Newton iteration.
+//
+// Mixed precision for models linear in u (reliable_delta > 0): the tangent
+// is assembled once in FP32 and a single FP32 Jacobi-PCG runs across Newton
+// iterations; whenever its recursive residual has dropped by reliable_delta,
+// the FP32 correction is added to the FP64 DOFs, the FP64 residual is
+// assembled (the Newton test) and the CG residual is replaced by it while the
+// search direction is kept.  That is FP64 iterative refinement without
+// discarding the Krylov space at each refinement step.
 #include <chrono>
 #include <cmath>
 
@@ -19,73 +27,62 @@
 
 namespace apx {
 
-void newton(Plan& P, double* u, const double* dvals, const apx_newton_options& o, apx_newton_stats& s,
-            hipStream_t st) {
-  APX_REQUIRE(o.precond == APX_PRECOND_JACOBI || o.precond == APX_PRECOND_NONE, APX_ERR_UNSUPPORTED,
-              "apx_newton: preconditioner not available in this entry point");
-  const auto t_start = std::chrono::steady_clock::now();
-  auto secs = [](std::chrono::steady_clock::time_point a) {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-  };
-  const int prec = o.precision;
-  const size_t tsz = prec == APX_FP64 ? 8 : 4;
-  if (P.r_full.n < (size_t)P.n_dofs) P.r_full.alloc(P.n_dofs);
-  // vectors: inv_diag, b, x (n_free each) in the Krylov precision
-  DevBuf<char> vecs;
-  vecs.alloc(3 * P.n_free * tsz);
-  void* inv = vecs.p;
-  void* b = vecs.p + P.n_free * tsz;
-  void* x = vecs.p + 2 * P.n_free * tsz;
-  void* vals;
-  if (prec == APX_FP64) {
-    if (P.vals64.n < (size_t)P.nnz) P.vals64.alloc(P.nnz);
-    vals = P.vals64.p;
-  } else {
-    if (P.vals32.n < (size_t)P.nnz) P.vals32.alloc(P.nnz);
-    vals = P.vals32.p;
-  }
-  double* r = P.r_full.p;
+namespace {
+struct NewtonCtx {
+  Plan& P;
+  double* u;
+  double* r;
+  hipStream_t st;
+  std::vector<EventPair> ev_res, ev_tan;
+  apx_newton_stats& s;
 
-  s = apx_newton_stats{};
-  impose_dirichlet(P, u, dvals, st);
-  auto t0 = std::chrono::steady_clock::now();
-  assemble_residual(P, u, r, st);
-  const double res0 = free_norm(P, r, st);
-  s.seconds_residual += secs(t0);
-  s.initial_residual = res0;
-  const double thr = o.relative ? o.newton_tol * res0 : o.newton_tol;
-
-  int itt = 0;
-  double res_old = 0.0, res = res0;
-  bool not_stop = true, diverged = false, have_tangent = false;
-  while (not_stop) {
-    if (!have_tangent || !o.reuse_tangent) {
-      t0 = std::chrono::steady_clock::now();
-      assemble_tangent(P, u, vals, prec, st);
-      if (o.precond == APX_PRECOND_JACOBI) {
-        jacobi_inverse(P, vals, inv, prec, st);
-      } else if (prec == APX_FP64) {
-        fill<double>((double*)inv, P.n_free, 1.0, st);
-      } else {
-        fill<float>((float*)inv, P.n_free, 1.0f, st);
-      }
-      APX_HIP(hipStreamSynchronize(st));
-      s.seconds_tangent += secs(t0);
-      have_tangent = true;
-    }
-    gather_rhs(P, r, b, prec, st);
-    t0 = std::chrono::steady_clock::now();
-    apx_krylov_stats ks = pcg(P, vals, inv, b, x, prec, o.lin_tol, o.lin_atol, o.lin_maxiter, st);
-    s.seconds_krylov += secs(t0);
-    s.krylov_iterations += ks.iterations;
-    scatter_update(P, u, x, o.damping, prec, st);
-    t0 = std::chrono::steady_clock::now();
+  void residual() {
+    auto t0 = std::chrono::steady_clock::now();
+    EventPair e = timing_begin(P, st);
     assemble_residual(P, u, r, st);
-    res = free_norm(P, r, st);
-    s.seconds_residual += secs(t0);
-    not_stop = res > thr;
+    timing_end(P, e, st);
+    if (P.timing) ev_res.push_back(e);
+    s.seconds_residual += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  double norm() {
+    auto t0 = std::chrono::steady_clock::now();
+    double v = free_norm(P, r, st);
+    s.seconds_residual += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return v;
+  }
+  void tangent(void* vals, void* inv, int prec, int precond) {
+    auto t0 = std::chrono::steady_clock::now();
+    EventPair e = timing_begin(P, st);
+    assemble_tangent(P, u, vals, prec, st);
+    timing_end(P, e, st);
+    if (P.timing) ev_tan.push_back(e);
+    if (precond == APX_PRECOND_JACOBI)
+      jacobi_inverse(P, vals, inv, prec, st);
+    else if (prec == APX_FP64)
+      fill<double>((double*)inv, P.n_free, 1.0, st);
+    else
+      fill<float>((float*)inv, P.n_free, 1.0f, st);
+    APX_HIP(hipStreamSynchronize(st));
+    s.seconds_tangent += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  void finish(int prec) {
+    APX_HIP(hipStreamSynchronize(st));
+    timing_collect(P, "residual", ev_res, residual_bytes(P), ev_res.size());
+    timing_collect(P, prec == APX_FP64 ? "tangent_f64" : "tangent_f32", ev_tan, tangent_bytes(P, prec),
+                   ev_tan.size());
+  }
+};
+
+// Newton bookkeeping shared by both drivers (solver.py:898-930).
+struct NewtonTest {
+  int itt = 0;
+  double res_old = 0.0;
+  bool diverged = false;
+  // returns true when the loop continues
+  bool step(double res, double thr, int maxiter, apx_newton_stats& s) {
+    const bool not_stop = res > thr;
     bool next;
-    if (itt < o.maxiter) {
+    if (itt < maxiter) {
       diverged = (itt > 1 && res / res_old > 10.0) || !std::isfinite(res);
       next = !diverged;
     } else {
@@ -95,13 +92,126 @@ void newton(Plan& P, double* u, const double* dvals, const apx_newton_options& o
     if (itt < 64) s.history[itt] = res;
     ++itt;
     res_old = res;
-    not_stop = not_stop && next;
+    return not_stop && next;
   }
-  APX_HIP(hipStreamSynchronize(st));
-  s.iterations = itt;
-  s.diverged = diverged ? 1 : 0;
+};
+}  // namespace
+
+template <class T>
+static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_options& o) {
+  Plan& P = C.P;
+  apx_newton_stats& s = C.s;
+  const int prec = sizeof(T) == 8 ? APX_FP64 : APX_FP32;
+  DevBuf<T> vecs;
+  vecs.alloc(2 * P.n_free);
+  T* inv = vecs.p;
+  T* b = vecs.p + P.n_free;
+  T* vals;
+  if constexpr (sizeof(T) == 8) {
+    if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
+    vals = P.vals64.p;
+  } else {
+    if (P.vals32.n < (size_t)P.sell_total) P.vals32.alloc(P.sell_total);
+    vals = P.vals32.p;
+  }
+  impose_dirichlet(P, C.u, dvals, C.st);
+  C.residual();
+  const double res0 = C.norm();
+  s.initial_residual = res0;
+  const double thr = o.relative ? o.newton_tol * res0 : o.newton_tol;
+
+  NewtonTest test;
+  double res = res0;
+  bool have_tangent = false, go = true;
+  PcgRun<T> pcg(P, vals, inv, C.st);
+  while (go) {
+    if (!have_tangent || !o.reuse_tangent) {
+      C.tangent(vals, inv, prec, o.precond);
+      have_tangent = true;
+    }
+    gather_rhs(P, C.r, b, prec, C.st);
+    double eta = o.lin_tol;
+    if (o.forcing && res > 0.0) eta = std::max(eta, std::min(0.5, 0.5 * thr / res));
+    auto t0 = std::chrono::steady_clock::now();
+    pcg.init(b, nullptr, eta, o.lin_atol, o.lin_maxiter, 0.0);
+    pcg.iterations = 0;
+    pcg.counted = 0;
+    KState hs = pcg.run();
+    pcg.collect_timing();
+    s.seconds_krylov += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    s.krylov_iterations += hs.k;
+    scatter_update(P, C.u, pcg.x, o.damping, prec, C.st);
+    C.residual();
+    res = C.norm();
+    go = test.step(res, thr, o.maxiter, s);
+  }
+  s.iterations = test.itt;
+  s.diverged = test.diverged ? 1 : 0;
   s.residual_norm = res;
-  s.seconds_total = secs(t_start);
+}
+
+static void newton_reliable(NewtonCtx& C, const double* dvals, const apx_newton_options& o) {
+  Plan& P = C.P;
+  apx_newton_stats& s = C.s;
+  DevBuf<float> vecs;
+  vecs.alloc(2 * P.n_free);
+  float* inv = vecs.p;
+  float* b = vecs.p + P.n_free;
+  if (P.vals32.n < (size_t)P.sell_total) P.vals32.alloc(P.sell_total);
+  float* vals = P.vals32.p;
+  impose_dirichlet(P, C.u, dvals, C.st);
+  C.residual();
+  const double res0 = C.norm();
+  s.initial_residual = res0;
+  const double thr = o.relative ? o.newton_tol * res0 : o.newton_tol;
+  C.tangent(vals, inv, APX_FP32, o.precond);
+  gather_rhs(P, C.r, b, APX_FP32, C.st);
+  auto t0 = std::chrono::steady_clock::now();
+  PcgRun<float> pcg(P, vals, inv, C.st);
+  // the FP32 recursion stops on its own at half the Newton threshold; the
+  // FP64 residual decides at every anchor
+  pcg.init(b, nullptr, 0.0, 0.5 * thr, o.lin_maxiter, o.reliable_delta);
+  NewtonTest test;
+  double res = res0;
+  bool go = true;
+  while (go) {
+    KState hs = pcg.run();
+    pcg.collect_timing();
+    scatter_update(P, C.u, pcg.x, 1.0, APX_FP32, C.st);
+    C.residual();
+    res = C.norm();
+    go = test.step(res, thr, o.maxiter, s);
+    if (go && hs.k >= pcg.maxiter) {
+      test.diverged = true;
+      go = false;
+    }
+    if (go) pcg.reanchor(C.r);
+  }
+  s.seconds_krylov += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  s.krylov_iterations = pcg.iterations;
+  s.iterations = test.itt;
+  s.diverged = test.diverged ? 1 : 0;
+  s.residual_norm = res;
+}
+
+void newton(Plan& P, double* u, const double* dvals, const apx_newton_options& o, apx_newton_stats& s,
+            hipStream_t st) {
+  APX_REQUIRE(o.precond == APX_PRECOND_JACOBI || o.precond == APX_PRECOND_NONE, APX_ERR_UNSUPPORTED,
+              "apx_newton: preconditioner not available in this entry point");
+  const auto t_start = std::chrono::steady_clock::now();
+  if (P.r_full.n < (size_t)P.n_dofs) P.r_full.alloc(P.n_dofs);
+  s = apx_newton_stats{};
+  NewtonCtx C{P, u, P.r_full.p, st, {}, {}, s};
+  if (o.precision == APX_FP32 && o.reuse_tangent && o.reliable_delta > 0.0) {
+    APX_REQUIRE(o.damping == 1.0, APX_ERR_ARG, "reliable-update refinement needs damping 1");
+    newton_reliable(C, dvals, o);
+  } else if (o.precision == APX_FP64) {
+    newton_standard<double>(C, dvals, o);
+  } else {
+    newton_standard<float>(C, dvals, o);
+  }
+  C.finish(o.precision);
+  s.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 
 }  // namespace apx

@@ -17,11 +17,59 @@ Plan::~Plan() {
   if (pinned) (void)hipHostFree(pinned);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  for (auto e : evpool) (void)hipEventDestroy(e);
+}
+
+static hipEvent_t pool_event(Plan& P) {
+  if (!P.evpool.empty()) {
+    hipEvent_t e = P.evpool.back();
+    P.evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  APX_HIP(hipEventCreate(&e));
+  return e;
+}
+
+EventPair timing_begin(Plan& P, hipStream_t st) {
+  EventPair e;
+  if (!P.timing) return e;
+  e.a = pool_event(P);
+  e.b = pool_event(P);
+  APX_HIP(hipEventRecord(e.a, st));
+  return e;
+}
+
+void timing_end(Plan& P, EventPair& e, hipStream_t st) {
+  if (e.b) APX_HIP(hipEventRecord(e.b, st));
+}
+
+void timing_collect(Plan& P, const char* name, std::vector<EventPair>& pairs, int64_t bytes, size_t count_limit) {
+  if (pairs.empty()) return;
+  auto& t = P.timers[name];
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    auto& e = pairs[i];
+    if (!e.a) continue;
+    if (i < count_limit) {
+      APX_HIP(hipEventSynchronize(e.b));
+      float ms = 0.f;
+      APX_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+      t.total_ms += ms;
+      t.count += 1;
+      t.bytes = bytes;
+    } else {
+      APX_HIP(hipEventSynchronize(e.b));
+    }
+    P.evpool.push_back(e.a);
+    P.evpool.push_back(e.b);
+  }
+  pairs.clear();
 }
 
 int64_t Plan::device_bytes() const {
   int64_t b = coords.bytes() + dmask.bytes() + full2red.bytes() + red2full.bytes() + indptr.bytes() +
-              indices.bytes() + diag_slot.bytes() + vals64.bytes() + vals32.bytes() + w64.bytes() +
+              indices.bytes() + diag_slot.bytes() + sell_ptr.bytes() + sell_cols.bytes() + csr2sell.bytes() +
+              vals64.bytes() + vals32.bytes() + w64.bytes() +
               w32.bytes() + r_full.bytes() + diag_full.bytes() + partials.bytes();
   for (auto& s : sets) b += s.conn.bytes() + s.slots.bytes();
   return b;
@@ -99,7 +147,31 @@ void plan_build_host(Plan& P, std::vector<std::vector<int32_t>>& conn_perm,
       }
   }
 
-  // 4. greedy element colouring per set (node-disjoint colours) and slot maps
+  // 4. SELL-64 layout of the reduced matrix
+  P.n_chunks = (P.n_free + kSellC - 1) / kSellC;
+  P.sell_ptr_h.assign(P.n_chunks + 1, 0);
+  for (int64_t c = 0; c < P.n_chunks; ++c) {
+    int32_t w = 0;
+    for (int64_t r = c * kSellC; r < std::min<int64_t>(P.n_free, (c + 1) * kSellC); ++r)
+      w = std::max(w, P.indptr_h[r + 1] - P.indptr_h[r]);
+    APX_REQUIRE((int64_t)P.sell_ptr_h[c] + (int64_t)w * kSellC < (int64_t)INT32_MAX, APX_ERR_UNSUPPORTED,
+                "SELL layout exceeds int32 indexing");
+    P.sell_ptr_h[c + 1] = P.sell_ptr_h[c] + w * kSellC;
+  }
+  P.sell_total = P.sell_ptr_h[P.n_chunks];
+  P.sell_cols_h.assign(P.sell_total, 0);
+  P.csr2sell_h.assign(P.nnz, 0);
+  for (int64_t r = 0; r < P.n_free; ++r) {
+    const int64_t c = r / kSellC, lane = r % kSellC;
+    for (int32_t k = P.indptr_h[r]; k < P.indptr_h[r + 1]; ++k) {
+      const int64_t s = P.sell_ptr_h[c] + (int64_t)(k - P.indptr_h[r]) * kSellC + lane;
+      P.sell_cols_h[s] = P.indices_h[k];
+      P.csr2sell_h[k] = (int32_t)s;
+    }
+  }
+
+  // 5. greedy element colouring per set (node-disjoint colours) and slot maps
+  //    (slots index the SELL layout)
   conn_perm.assign(P.sets.size(), {});
   slots_perm.assign(P.sets.size(), {});
   std::vector<uint64_t> used(N);
@@ -143,7 +215,7 @@ void plan_build_host(Plan& P, std::vector<std::vector<int32_t>>& conn_perm,
             const int32_t* e = P.indices_h.data() + P.indptr_h[row + 1];
             const int32_t* f = std::lower_bound(b, e, col);
             APX_REQUIRE(f != e && *f == col, APX_ERR_INTERNAL, "slot lookup failed");
-            slot = (int32_t)(f - P.indices_h.data());
+            slot = P.csr2sell_h[f - P.indices_h.data()];
           }
           sl[((size_t)i * nd + A) * nd + B] = slot;
         }
@@ -173,9 +245,14 @@ void plan_finalize(Plan& P) {
     const int32_t* e = P.indices_h.data() + P.indptr_h[r + 1];
     const int32_t* f = std::lower_bound(b, e, (int32_t)r);
     APX_REQUIRE(f != e && *f == r, APX_ERR_INTERNAL, "missing diagonal entry in reduced CSR");
-    ds[r] = (int32_t)(f - P.indices_h.data());
+    ds[r] = P.csr2sell_h[f - P.indices_h.data()];
   }
   P.diag_slot.upload(ds.data(), ds.size());
+  P.sell_ptr.upload(P.sell_ptr_h.data(), P.sell_ptr_h.size());
+  P.sell_cols.upload(P.sell_cols_h.data(), P.sell_cols_h.size());
+  P.csr2sell.upload(P.csr2sell_h.data(), P.csr2sell_h.size());
+  std::vector<int32_t>().swap(P.sell_cols_h);
+  std::vector<int32_t>().swap(P.csr2sell_h);
   for (size_t si = 0; si < P.sets.size(); ++si) {
     P.sets[si].conn.upload(conn_perm[si].data(), conn_perm[si].size());
     P.sets[si].slots.upload(slots_perm[si].data(), slots_perm[si].size());

@@ -54,27 +54,50 @@ struct Plan {
   DevBuf<int32_t> red2full;    // n_free
   DevBuf<int32_t> indptr;      // n_free + 1
   DevBuf<int32_t> indices;     // nnz
-  DevBuf<int32_t> diag_slot;   // n_free
+  // Internal matrix layout: SELL-64 (chunks of 64 rows = one wavefront; within
+  // a chunk entries are stored column-major, lane = row, padded to the chunk's
+  // longest row).  Element slot maps point into this layout; the C ABI
+  // exchanges values in canonical CSR order through csr2sell.
+  int64_t sell_total = 0, n_chunks = 0;
+  std::vector<int32_t> sell_ptr_h, sell_cols_h, csr2sell_h;
+  DevBuf<int32_t> sell_ptr;    // n_chunks + 1 (entry offsets)
+  DevBuf<int32_t> sell_cols;   // sell_total (padding: column 0, value 0)
+  DevBuf<int32_t> csr2sell;    // nnz
+  DevBuf<int32_t> diag_slot;   // n_free, position of the diagonal in SELL
 
   // workspace (lazily sized)
   DevBuf<double> vals64, w64;  // nnz, vectors
   DevBuf<float> vals32, w32;
   DevBuf<double> r_full, diag_full;  // n_dofs
   DevBuf<double> partials;     // reduction partials
-  DevBuf<unsigned int> counters;
   DevBuf<double> kstate;       // device Krylov scalars
   double* pinned = nullptr;    // host pinned scalars
   bool timing = false;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> evpool;  // recycled timing events
 
   ~Plan();
   int64_t device_bytes() const;
 };
 
+// Non-blocking launch timing: event pairs recorded on the launch stream and
+// resolved after the work completes.
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+EventPair timing_begin(Plan& P, hipStream_t st);
+void timing_end(Plan& P, EventPair& e, hipStream_t st);
+// Wait for the pairs, add their durations to timer `name`, recycle events.
+void timing_collect(Plan& P, const char* name, std::vector<EventPair>& pairs, int64_t bytes, size_t count_limit);
+
 inline int64_t ndofs_elem(const Plan& P, const ElemSet& s) { return (int64_t)s.npe * P.d; }
+
+constexpr int kSellC = 64;
 
 // plan.hip
 void plan_finalize(Plan& P);
+void plan_build_host(Plan& P, std::vector<std::vector<int32_t>>& conn_perm,
+                     std::vector<std::vector<int32_t>>& slots_perm);
 
 }  // namespace apx

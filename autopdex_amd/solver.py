@@ -20,6 +20,13 @@ new keys of this backend:
   'linear atol', 'linear maxiter'
   'newton tolerance mode'  'absolute' (reference, P6) | 'relative'
                            (tol * ||r_free(u0)||_2)
+  'linear forcing'         True: Krylov tolerance max(linear tolerance,
+                           0.5 tol/||r_k||) -- no inner solve tighter than the
+                           Newton test needs (default False: fixed tolerance)
+  'reliable delta'         mixed precision on models linear in u: one FP32 CG
+                           re-anchored to the FP64 residual each time its
+                           residual drops by this factor (default 0.1; 0 turns
+                           it off and restarts CG at every Newton step)
 Newton semantics (P4-P6) are implemented in the device driver apx_newton
 (solver.py:740-948).
 """
@@ -85,6 +92,8 @@ def newton_options(static_settings, newton_tol=1e-8, maxiter=30, damping=1.0, re
                             (N.MODEL_POISSON_SOURCE, N.MODEL_POISSON_VARK, N.MODEL_LINE_TRACTION)
                             for m in static_settings["model"])
     o.reuse_tangent = 1 if reuse_tangent else 0
+    o.forcing = 1 if _opt(static_settings, "linear forcing", False) else 0
+    o.reliable_delta = float(_opt(static_settings, "reliable delta", 0.1)) if prec == "mixed" else 0.0
     return o
 
 

@@ -142,6 +142,12 @@ typedef struct apx_newton_options {
   double lin_atol;
   int lin_maxiter;          /* <= 0: 10 * n_free (jax cg default)                    */
   int reuse_tangent;        /* 1: model is linear in u, assemble the tangent once    */
+  int forcing;              /* 1: Krylov tolerance max(lin_tol, 0.5 thr/||r_k||), so no
+                               solve is tighter than the Newton test needs           */
+  double reliable_delta;    /* > 0 with precision FP32 and reuse_tangent: one FP32 CG
+                               whose residual is re-anchored to the FP64 residual
+                               whenever it drops by this factor (reliable updates);
+                               each anchor is a Newton iteration                      */
 } apx_newton_options;
 
 typedef struct apx_newton_stats {
