@@ -1,0 +1,276 @@
+"""Pins for the CPU oracle (no GPU).
+
+Each oracle function is checked against something it does not restate:
+the reference's own known-answer test, closed forms, finite differences,
+an automatic differentiation of the reference's element potential, patch
+tests, brute force.
+"""
+import numpy as np
+import pytest
+
+from oracle import assembly as OA
+from oracle import elements as OE
+from oracle import mesh as OM
+from oracle import newton as ON
+from tests import problems as P
+
+UNIT = [[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]]
+
+
+# ---- reference known-answer test ----------------------------------------------------------
+
+def test_reference_golden_5x5():
+    """tests/test_dicts_as_dofs_user_potential.py:63 -- sum(phi) = 1.9066412530282952."""
+    coords, elems = OM.structured_mesh((5, 5), UNIT)
+    mask = OM.in_sdfs(coords, lambda x: OM.psdf_polygon(x, UNIT))
+    sets = [(OE.PoissonSource(), elems, None)]
+    sol, info = ON.newton(np.zeros(36), lambda u: OA.residual(sets, coords, u, 1),
+                          lambda u: OA.reduced_tangent(sets, coords, u, 1, ~mask), mask, np.zeros(36),
+                          lambda A, b: ON.linear_solve(A, b)[0])
+    assert abs(sol.sum() - 1.9066412530282952) < 1e-14
+    assert info[0] == 1 and info[2] is False
+
+
+def test_corner_nodes_are_free_and_mask_counts():
+    """P1: psdf_polygon is NaN at the corners -> not Dirichlet.  Counts at 200x200."""
+    coords, _ = OM.structured_mesh((200, 200), UNIT)
+    mask = OM.in_sdfs(coords, lambda x: OM.psdf_polygon(x, UNIT))
+    on_edge = np.isclose(coords[:, 0], 0) | np.isclose(coords[:, 0], 1) | np.isclose(coords[:, 1], 0) | \
+        np.isclose(coords[:, 1], 1)
+    corner = (np.isclose(coords[:, 0], 0) | np.isclose(coords[:, 0], 1)) & \
+        (np.isclose(coords[:, 1], 0) | np.isclose(coords[:, 1], 1))
+    assert np.array_equal(mask, on_edge & ~corner)
+    assert mask.sum() == 796 and (~mask).sum() == 39605
+
+
+# ---- mesh / quadrature closed forms -----------------------------------------------------------
+
+def test_structured_mesh_closed_form():
+    coords, elems = OM.structured_mesh((3, 4), [[0, 0], [2, 0], [2, 1], [0, 1]])
+    i, j = 2, 3
+    assert np.allclose(coords[i * 5 + j], [2 * i / 3, j / 4])
+    assert list(elems[i * 4 + 1]) == [i * 5 + 1, i * 5 + 2, (i + 1) * 5 + 2, (i + 1) * 5 + 1]
+    c3, e3 = OM.structured_mesh((2, 2, 2), P.UNIT_CUBE, "brick")
+    b = 1 * 9 + 1 * 3 + 1
+    assert list(e3[7]) == [b, b + 9, b + 12, b + 3, b + 1, b + 10, b + 13, b + 4]
+    assert np.allclose(c3[b], [0.5, 0.5, 0.5])
+
+
+def test_gauss_rule_closed_form():
+    x, w = OM.gauss_legendre_nd(2, 2)
+    g = 1 / np.sqrt(3)
+    assert np.allclose(x, [[-g, -g], [g, -g], [-g, g], [g, g]], atol=2e-16) and np.all(w == 1.0)
+    x3, w3 = OM.gauss_legendre_nd(3, 2)
+    assert np.allclose(x3[5], [g, -g, g], atol=2e-16) and np.all(w3 == 1.0)
+    # exact for cubic polynomials in 1D: int_{-1}^{1} x^2 = 2/3
+    x1, w1 = OM.gauss_legendre_nd(1, 2)
+    assert abs(np.sum(w1 * x1 ** 2) - 2 / 3) < 1e-15 and abs(np.sum(w1 * x1 ** 3)) < 1e-15
+
+
+@pytest.mark.parametrize("npe,dim", [(4, 2), (8, 3)])
+def test_shape_functions_partition_kronecker_gradient(npe, dim):
+    nodes = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]]) if dim == 2 else OE.HEX8_SIGNS
+    for a in range(npe):
+        N, _ = OE.shape(npe, dim, nodes[a])
+        assert np.allclose(N, np.eye(npe)[a])
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        xi = rng.uniform(-1, 1, dim)
+        N, dN = OE.shape(npe, dim, xi)
+        assert abs(N.sum() - 1) < 1e-15 and np.allclose(dN.sum(0), 0, atol=1e-15)
+        # gradient by central differences
+        for k in range(dim):
+            e = np.zeros(dim)
+            e[k] = 1e-6
+            fd = (OE.shape(npe, dim, xi + e)[0] - OE.shape(npe, dim, xi - e)[0]) / 2e-6
+            assert np.allclose(fd, dN[:, k], atol=1e-9)
+
+
+def test_q1_mesher_orientation_negative_det():
+    """P2: structured quads are clockwise in (xi0, xi1): det J < 0; bricks positive."""
+    coords, elems = OM.structured_mesh((2, 2), UNIT)
+    det, _ = OE.geometry(coords[elems], OE.q1_shape(np.zeros(2))[1])
+    assert np.all(det < 0) and np.allclose(det, -0.0625)
+    c3, e3 = OM.structured_mesh((2, 2, 2), P.UNIT_CUBE, "brick")
+    det3, _ = OE.geometry(c3[e3], OE.hex8_shape(np.zeros(3))[1])
+    assert np.all(det3 > 0)
+
+
+# ---- element derivatives: finite differences -----------------------------------------------------
+
+def _elements(builder, *a):
+    dofs, settings, static, o = builder(*a)
+    model, conn, nodal = o["sets"][0]
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(o["mask"].size) * (0.05 if o["d"] > 1 else 1.0)
+    xe, ue, ne = OA.gather(o["coords"], conn[:6], u, o["d"], nodal)
+    return model, xe, ue, ne, o["d"]
+
+
+@pytest.mark.parametrize("builder,args", [(P.poisson_source, (4,)), (P.poisson_vark, (4, 1)),
+                                          (P.neo_hooke_cube, (2,)), (P.tapered_panel, (4,))])
+def test_residual_and_tangent_are_derivatives(builder, args):
+    model, xe, ue, ne, d = _elements(builder, *args)
+    r = model.residual(xe, ue, ne).reshape(len(xe), -1)
+    K = model.tangent(xe, ue, ne)
+    flat = ue.reshape(len(xe), -1)
+    h = 1e-6
+    for a in range(flat.shape[1]):
+        dp, dm = flat.copy(), flat.copy()
+        dp[:, a] += h
+        dm[:, a] -= h
+        pot = (model.potential(xe, dp.reshape(ue.shape), ne) - model.potential(xe, dm.reshape(ue.shape), ne)) / (2 * h)
+        assert np.allclose(pot, r[:, a], rtol=1e-6, atol=1e-8 * np.abs(r).max())
+        col = (model.residual(xe, dp.reshape(ue.shape), ne).reshape(len(xe), -1)
+               - model.residual(xe, dm.reshape(ue.shape), ne).reshape(len(xe), -1)) / (2 * h)
+        assert np.allclose(col, K[:, :, a], rtol=1e-6, atol=1e-8 * np.abs(K).max())
+    assert np.allclose(K, np.swapaxes(K, 1, 2), atol=1e-12 * np.abs(K).max())
+
+
+def test_poisson_tangent_rows_sum_to_zero():
+    model, xe, ue, ne, d = _elements(P.poisson_source, 4)
+    K = model.tangent(xe, ue, ne)
+    assert np.allclose(K.sum(2), 0, atol=1e-14)
+
+
+# ---- independent automatic differentiation of the reference potential ----------------------------
+
+def _torch_potential(torch, xe, ue, kind, params):
+    """The reference's potential (models.py:1188-1269): sum_q w det(dX/dxi) psi with the physical
+    gradient du/dxi (dX/dxi)^-1 (spaces.py:12162-12191), evaluated with torch float64 and
+    differentiated by torch.autograd -- an AD path independent of the oracle's closed forms."""
+    dim = xe.shape[-1]
+    xq, wq = OM.gauss_legendre_nd(dim, 2)
+    total = 0
+    for q in range(len(wq)):
+        N, dN = OE.shape(xe.shape[1], dim, xq[q])
+        N, dN = torch.tensor(N), torch.tensor(dN)
+        J = torch.einsum("eai,aj->eij", xe, dN)
+        du = torch.einsum("ea...,aj->e...j", ue, dN)  # d u / d xi
+        grad = torch.einsum("e...j,ejk->e...k", du, torch.linalg.inv(J))
+        det = torch.linalg.det(J)
+        if kind == "poisson":
+            x = torch.einsum("a,eai->ei", N, xe)
+            x2 = x - torch.tensor([1.0, 0.5], dtype=torch.float64)
+            b = 20 * (torch.sin(10 * (x * x).sum(-1)) - torch.cos(10 * (x2 * x2).sum(-1)))
+            psi = 0.5 * (grad * grad).sum(-1) - b * (ue @ N)
+        else:
+            lam, mu = params
+            F = torch.eye(dim, dtype=torch.float64) + grad
+            trC = (F * F).sum((-1, -2)) + (1.0 if dim == 2 else 0.0)
+            Jf = torch.linalg.det(F)
+            psi = mu / 2 * (trC - 3 - 2 * torch.log(Jf)) + lam / 4 * (Jf ** 2 - 1 - 2 * torch.log(Jf))
+        total = total + (wq[q] * det * psi).sum()
+    return total
+
+
+@pytest.mark.parametrize("builder,args,kind", [(P.poisson_source, (4,), "poisson"),
+                                               (P.neo_hooke_cube, (2,), "neo"),
+                                               (P.tapered_panel, (4,), "neo")])
+def test_closed_forms_match_autograd_of_reference_potential(builder, args, kind):
+    torch = pytest.importorskip("torch")
+    model, xe, ue, ne, d = _elements(builder, *args)
+    params = (getattr(model, "lam", 0.0), getattr(model, "mu", 0.0))
+    X = torch.tensor(xe)
+    U = torch.tensor(ue, requires_grad=True)
+    pot = _torch_potential(torch, X, U, kind, params)
+    (g,) = torch.autograd.grad(pot, U, create_graph=True)
+    r_ad = g.detach().numpy().reshape(len(xe), -1)
+    r = model.residual(xe, ue, ne).reshape(len(xe), -1)
+    assert np.allclose(r_ad, r, rtol=1e-12, atol=1e-13 * np.abs(r).max())
+    gf = g.reshape(len(xe), -1)
+    n = gf.shape[1]
+    K_ad = np.zeros((len(xe), n, n))
+    for a in range(n):
+        (h,) = torch.autograd.grad(gf[:, a].sum(), U, retain_graph=True)
+        K_ad[:, a, :] = h.numpy().reshape(len(xe), -1)
+    K = model.tangent(xe, ue, ne)
+    assert np.allclose(K_ad, K, rtol=1e-11, atol=1e-12 * np.abs(K).max())
+
+
+# ---- Neo-Hooke invariants --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_neo_hooke_reference_state_and_linear_limit(n):
+    lam, mu = 10.0, 1.0
+    F = np.eye(n)[None]
+    assert abs(OE.neo_hooke_psi(F, lam, mu)[0]) < 1e-15
+    assert np.allclose(OE.neo_hooke_stress(F, lam, mu), 0)
+    A = OE.neo_hooke_moduli(F, lam, mu)[0]
+    I = np.eye(n)
+    C = lam * np.einsum("ij,kl->ijkl", I, I) + mu * (np.einsum("ik,jl->ijkl", I, I) + np.einsum("il,jk->ijkl", I, I))
+    assert np.allclose(A, C)
+    # objectivity: psi(QF) = psi(F)
+    rng = np.random.default_rng(2)
+    Fr = np.eye(n) + 0.1 * rng.standard_normal((n, n))
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    assert abs(OE.neo_hooke_psi((Q @ Fr)[None], lam, mu)[0] - OE.neo_hooke_psi(Fr[None], lam, mu)[0]) < 1e-13
+
+
+def test_patch_tests_zero_interior_residual():
+    """A linear field is reproduced exactly: Poisson K u = 0 and a homogeneous Neo-Hooke deformation
+    has zero internal force at interior nodes."""
+    coords, elems = OM.structured_mesh((4, 4), [[0, 0], [2, 0.3], [2.2, 1.4], [0.1, 1]])
+    interior = np.ones(len(coords), bool)
+    interior[np.unique(np.concatenate([elems[:, 0], elems[:, 1], elems[:, 2], elems[:, 3]]))] = True
+    boundary = np.isin(np.arange(len(coords)), np.concatenate([np.arange(5), np.arange(20, 25),
+                                                                np.arange(0, 25, 5), np.arange(4, 25, 5)]))
+    phi = 0.3 + coords @ np.array([1.5, -0.7])
+    K = OA.dense_bruteforce(OE.PoissonSource(), coords, elems, phi, 1)
+    assert np.allclose((K @ phi)[~boundary], 0, atol=1e-12)
+    G = np.array([[0.05, 0.02], [-0.01, 0.03]])
+    u = coords @ G.T
+    r = OA.residual([(OE.NeoHooke(40.0, 0.4), elems, None)], coords, u.ravel(), 2).reshape(-1, 2)
+    assert np.allclose(r[~boundary], 0, atol=1e-12)
+
+
+def test_line_traction_total_force():
+    coords, elems = OM.structured_mesh((4, 6), P.TAPERED)
+    from autopdex_amd import mesher
+    lines = mesher.boundary_lines((4, 6), "right")
+    t = np.array([0.0, 1.0 / 16])
+    r = OA.residual([(OE.LineTraction(t), lines, None)], coords, np.zeros(2 * len(coords)), 2).reshape(-1, 2)
+    assert np.allclose(r.sum(0), -t * 16.0, atol=1e-15)
+
+
+# ---- assembly and solvers ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("builder,args", [(P.poisson_source, (5,)), (P.tapered_panel, (3,)), (P.neo_hooke_cube, (2,))])
+def test_reduced_csr_equals_bruteforce_dense(builder, args):
+    dofs, settings, static, o = builder(*args)
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(o["mask"].size) * 0.05
+    A = OA.reduced_tangent(o["sets"], o["coords"], u, o["d"], ~o["mask"]).toarray()
+    K = sum(OA.dense_bruteforce(m, o["coords"], c, u, o["d"], nd) for m, c, nd in o["sets"])
+    free = ~o["mask"]
+    assert np.allclose(A, K[np.ix_(free, free)], atol=1e-14 * np.abs(K).max())
+
+
+def test_pcg_matches_superlu_and_scipy_cg():
+    import scipy.sparse.linalg as spla
+    dofs, settings, static, o = P.poisson_source(30)
+    A = OA.reduced_tangent(o["sets"], o["coords"], np.zeros(o["mask"].size), 1, ~o["mask"])
+    b = np.random.default_rng(0).standard_normal(A.shape[0])
+    x_lu = spla.spsolve(A.tocsc(), b)
+    d = A.diagonal()
+    x, k = ON.pcg(A, b, lambda v: v / d, tol=1e-12)
+    assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) < 1e-9
+    # scipy.sparse.linalg.cg (same algorithm, independent implementation): same iteration count
+    count = []
+    xs, info = spla.cg(A, b, rtol=1e-12, M=spla.LinearOperator(A.shape, lambda v: v / d), callback=lambda _: count.append(1))
+    assert info == 0 and abs(len(count) - k) <= 1
+
+
+def test_newton_divergence_and_maxiter_semantics():
+    """P6: maxiter exhaustion returns diverged=True after maxiter+1 solves; NaN -> diverged."""
+    dofs, settings, static, o = P.neo_hooke_cube(2, disp=0.1)
+    sol, info = P.oracle_newton(o, tol=1e-30, maxiter=2)
+    assert info[0] == 3 and info[2] is True
+    from oracle import assembly as OA2
+    coords, sets, mask = o["coords"], o["sets"], o["mask"]
+    nan_res = lambda u: OA2.residual(sets, coords, u, 3) * np.nan
+    sol, info = ON.newton(np.zeros(mask.size), nan_res, lambda u: OA2.reduced_tangent(sets, coords, u, 3, ~mask),
+                          mask, o["values"], lambda A, b: ON.linear_solve(A, np.nan_to_num(b))[0])
+    assert info[2] is True and info[0] == 1
