@@ -60,6 +60,22 @@ _SIGS = {
     "apx_spmv": (_int, [_vp, _vp, _vp, _vp, _int, _vp]),
     "apx_pcg": (_int, [_vp, _vp, _vp, _vp, _int, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
     "apx_newton": (_int, [_vp, _vp, _vp, C.POINTER(NewtonOptions), C.POINTER(NewtonStats), _vp]),
+    "apx_unet_create": (_int, [C.POINTER(_vp), _int, _vp, _int, _int, _int, _int, _int, _int]),
+    "apx_unet_destroy": (_int, [_vp]),
+    "apx_unet_num_params": (_int, [_vp, C.POINTER(_i64), C.POINTER(_int)]),
+    "apx_unet_layer": (_int, [_vp, _int] + [C.POINTER(_i64)] * 2 + [C.POINTER(_int)] * 6),
+    "apx_unet_set_params": (_int, [_vp, _vp]),
+    "apx_unet_get_params": (_int, [_vp, _vp]),
+    "apx_unet_get_grads": (_int, [_vp, _vp]),
+    "apx_unet_buffers": (_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "apx_unet_forward": (_int, [_vp, _vp]),
+    "apx_unet_loss_backward": (_int, [_vp, _vp, C.POINTER(_dbl), _vp]),
+    "apx_unet_adam": (_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_float, _int, _vp]),
+    "apx_unet_load_pair": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "apx_unet_apply": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "apx_unet_train_step": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _dbl, C.POINTER(_dbl), _vp]),
+    "apx_plan_attach_unet": (_int, [_vp, _vp, _vp]),
+    "apx_fcg": (_int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
     "apx_set_timing": (_int, [_vp, _int]),
     "apx_get_timing": (_int, [_vp, C.c_char_p, C.POINTER(_dbl), C.POINTER(_i64), C.POINTER(_i64)]),
 }

@@ -255,3 +255,226 @@ int apx_get_timing(apx_plan* plan, const char* name, double* avg_ms, int64_t* co
 }
 
 }  // extern "C"
+
+// ---- learned preconditioner -------------------------------------------------------------
+#include "unet.hpp"
+
+struct apx_unet : public apx::UNet {};
+
+static inline UNet& netref(apx_unet* n) {
+  APX_REQUIRE(n != nullptr, APX_ERR_ARG, "null network");
+  return *n;
+}
+
+extern "C" {
+
+int apx_unet_create(apx_unet** out, int levels, const int* channels, int c_in, int c_out, int relu, int batch,
+                    int height, int width) {
+  return guarded([&] {
+    APX_REQUIRE(out && channels, APX_ERR_ARG, "null argument");
+    auto* n = new apx_unet();
+    try {
+      n->build(levels, channels, c_in, c_out, relu, batch, height, width);
+    } catch (...) {
+      delete n;
+      throw;
+    }
+    *out = n;
+  });
+}
+
+int apx_unet_destroy(apx_unet* net) {
+  return guarded([&] { delete net; });
+}
+
+int apx_unet_num_params(const apx_unet* net, int64_t* n_params, int* n_layers) {
+  return guarded([&] {
+    APX_REQUIRE(net, APX_ERR_ARG, "null network");
+    if (n_params) *n_params = net->n_params;
+    if (n_layers) *n_layers = (int)net->layers.size();
+  });
+}
+
+int apx_unet_layer(const apx_unet* net, int layer, int64_t* w_off, int64_t* b_off, int* ci, int* co, int* co_pad,
+                   int* k_pad, int* relu, int* level) {
+  return guarded([&] {
+    APX_REQUIRE(net && layer >= 0 && layer < (int)net->layers.size(), APX_ERR_ARG, "layer index out of range");
+    const NetLayer& ly = net->layers[layer];
+    if (w_off) *w_off = ly.w_off;
+    if (b_off) *b_off = ly.b_off;
+    if (ci) *ci = ly.ci;
+    if (co) *co = ly.co;
+    if (co_pad) *co_pad = ly.cop;
+    if (k_pad) *k_pad = ly.wk;
+    if (relu) *relu = ly.relu;
+    if (level) {
+      int lv = 0;
+      for (int l = 0; l + 1 < net->levels; ++l)
+        if (layer == net->L_down[l] || layer == net->L_up[l]) lv = l + 1;
+        else if (layer == net->L_e[l] || layer == net->L_d[l]) lv = l;
+      if (layer == net->L_b) lv = net->levels - 1;
+      *level = lv;
+    }
+  });
+}
+
+int apx_unet_set_params(apx_unet* net, const float* params_host) {
+  return guarded([&] {
+    UNet& N = netref(net);
+    APX_REQUIRE(params_host, APX_ERR_ARG, "null parameters");
+    APX_HIP(hipMemcpy(N.theta.p, params_host, N.n_params * sizeof(float), hipMemcpyHostToDevice));
+    APX_HIP(hipMemset(N.adam_m.p, 0, N.adam_m.bytes()));
+    APX_HIP(hipMemset(N.adam_v.p, 0, N.adam_v.bytes()));
+    N.prepare_weights(nullptr);
+    APX_HIP(hipDeviceSynchronize());
+  });
+}
+
+int apx_unet_get_params(const apx_unet* net, float* params_host) {
+  return guarded([&] {
+    APX_REQUIRE(net && params_host, APX_ERR_ARG, "null argument");
+    APX_HIP(hipDeviceSynchronize());
+    APX_HIP(hipMemcpy(params_host, net->theta.p, net->n_params * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int apx_unet_get_grads(const apx_unet* net, float* grads_host) {
+  return guarded([&] {
+    APX_REQUIRE(net && grads_host, APX_ERR_ARG, "null argument");
+    APX_HIP(hipDeviceSynchronize());
+    APX_HIP(hipMemcpy(grads_host, net->grad.p, net->n_params * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int apx_unet_buffers(apx_unet* net, void** x, void** y, void** scale) {
+  return guarded([&] {
+    UNet& N = netref(net);
+    if (x) *x = N.x0.p;
+    if (y) *y = N.y.p;
+    if (scale) *scale = N.scale.p;
+  });
+}
+
+int apx_unet_forward(apx_unet* net, void* stream) {
+  return guarded([&] { netref(net).forward(S(stream)); });
+}
+
+int apx_unet_loss_backward(apx_unet* net, const float* target_dev, double* loss, void* stream) {
+  return guarded([&] {
+    APX_REQUIRE(target_dev, APX_ERR_ARG, "null target");
+    const double l = netref(net).loss_backward(target_dev, S(stream));
+    if (loss) *loss = l;
+  });
+}
+
+int apx_unet_adam(apx_unet* net, float lr, float beta1, float beta2, float eps, int step, void* stream) {
+  return guarded([&] {
+    APX_REQUIRE(step >= 1, APX_ERR_ARG, "Adam step counts from 1");
+    netref(net).adam(lr, beta1, beta2, eps, step, S(stream));
+  });
+}
+
+int apx_unet_load_pair(apx_unet* net, apx_plan* plan, const double* r_dev, const double* g_nodal_dev,
+                       const double* e_dev, float* target_dev, int image, void* stream) {
+  return guarded([&] {
+    UNet& N = netref(net);
+    Plan& P = ready(plan);
+    APX_REQUIRE(r_dev, APX_ERR_ARG, "null residual");
+    net_input_from_reduced_img(P, N, r_dev, 1, g_nodal_dev, nullptr, image, S(stream));
+    if (e_dev) {
+      APX_REQUIRE(target_dev, APX_ERR_ARG, "null target buffer");
+      net_target_from_reduced_img(P, N, e_dev, target_dev, image, S(stream));
+    }
+  });
+}
+
+int apx_unet_apply(apx_unet* net, apx_plan* plan, const double* r_dev, const double* g_nodal_dev, double* z_dev,
+                   void* stream) {
+  return guarded([&] {
+    UNet& N = netref(net);
+    Plan& P = ready(plan);
+    APX_REQUIRE(r_dev && z_dev, APX_ERR_ARG, "null device pointer");
+    APX_REQUIRE(N.batch == 1, APX_ERR_ARG, "apply needs a batch-1 network");
+    net_input_from_reduced_img(P, N, r_dev, 1, g_nodal_dev, nullptr, 0, S(stream));
+    N.forward(S(stream));
+    net_output_to_reduced_img(P, N, z_dev, 1, 0, S(stream));
+  });
+}
+
+int apx_plan_attach_unet(apx_plan* plan, apx_unet* net, const double* g_nodal_dev) {
+  return guarded([&] {
+    Plan& P = ready(plan);
+    if (!net) {
+      P.net = nullptr;
+      P.net_g = nullptr;
+      return;
+    }
+    UNet& N = *net;
+    APX_REQUIRE(N.batch == 1, APX_ERR_ARG, "the preconditioner network must have batch 1");
+    APX_REQUIRE((int64_t)N.H[0] * N.W[0] == P.n_nodes, APX_ERR_ARG, "network grid does not match the plan's nodes");
+    APX_REQUIRE(N.c_out == P.d, APX_ERR_ARG, "network output channels must equal the DOFs per node");
+    APX_REQUIRE(N.c_in == P.d || (N.c_in == P.d + 1 && g_nodal_dev), APX_ERR_ARG,
+                "network input channels: DOFs per node, plus one when a nodal field is given");
+    // the grid convention: node (i, j) = i * width + j, Q1 connectivity of mesher.structured_mesh
+    APX_REQUIRE(P.dim == 2 && !P.sets.empty() && P.sets[0].npe == 4, APX_ERR_UNSUPPORTED,
+                "the learned preconditioner needs a structured 2D quad mesh");
+    const ElemSet& s0 = P.sets[0];
+    const int64_t nx = N.H[0] - 1, ny = N.W[0] - 1;
+    APX_REQUIRE(s0.n_elems == nx * ny, APX_ERR_ARG, "element count does not match the network grid");
+    for (int64_t e = 0; e < s0.n_elems; ++e) {
+      const int64_t i = e / ny, j = e % ny, b = i * (ny + 1) + j;
+      const int64_t* c = &s0.conn_host[e * 4];
+      APX_REQUIRE(c[0] == b && c[1] == b + 1 && c[2] == b + ny + 2 && c[3] == b + ny + 1, APX_ERR_ARG,
+                  "connectivity is not the structured grid of the network");
+    }
+    P.net = &N;
+    P.net_g = N.c_in > P.d ? g_nodal_dev : nullptr;
+  });
+}
+
+int apx_fcg(apx_plan* plan, const void* vals_dev, const double* b_dev, double* x_dev, int precision, double tol,
+            double atol, int maxiter, apx_krylov_stats* stats, void* stream) {
+  return guarded([&] {
+    Plan& P = ready(plan);
+    APX_REQUIRE(P.net, APX_ERR_STATE, "no network attached (apx_plan_attach_unet)");
+    APX_REQUIRE(vals_dev && b_dev && x_dev, APX_ERR_ARG, "null device pointer");
+    void* sell = sell_buffer(P, precision);
+    csr_to_sell(P, vals_dev, sell, precision, S(stream));
+    FcgResult r = precision == APX_FP64
+                      ? fcg_solve<double>(P, *P.net, (const double*)sell, P.net_g, b_dev, x_dev, tol, atol, maxiter,
+                                          S(stream))
+                      : fcg_solve<float>(P, *P.net, (const float*)sell, P.net_g, b_dev, x_dev, tol, atol, maxiter,
+                                         S(stream));
+    if (stats) {
+      stats->iterations = r.iterations;
+      stats->converged = r.converged ? 1 : 0;
+      stats->initial_residual = sqrt(r.rr0);
+      stats->final_residual = sqrt(r.rr);
+      stats->seconds = 0.0;
+    }
+  });
+}
+
+}  // extern "C"
+
+extern "C" int apx_unet_train_step(apx_unet* net, apx_plan* plan, const void* vals_dev, const double* const* r_list,
+                                   const double* const* g_list, const double* const* e_list, int loss_kind,
+                                   double w_e, double* loss, void* stream) {
+  return guarded([&] {
+    UNet& N = netref(net);
+    Plan& P = ready(plan);
+    APX_REQUIRE(vals_dev && r_list, APX_ERR_ARG, "null argument");
+    void* sell = sell_buffer(P, APX_FP64);
+    csr_to_sell(P, vals_dev, sell, APX_FP64, S(stream));
+    for (int b = 0; b < N.batch; ++b) {
+      APX_REQUIRE(r_list[b], APX_ERR_ARG, "null residual in batch");
+      net_input_from_reduced_img(P, N, r_list[b], 1, g_list ? g_list[b] : nullptr, nullptr, b, S(stream));
+    }
+    N.forward(S(stream));
+    APX_REQUIRE(loss_kind >= 0 && loss_kind <= 2, APX_ERR_ARG, "loss kind: 0 residual, 1 energy, 2 A-norm error");
+    APX_REQUIRE(loss_kind != 2 || e_list != nullptr, APX_ERR_ARG, "the A-norm loss needs e_list");
+    const double l =
+        unet_residual_loss_backward(P, N, (const double*)sell, r_list, e_list, w_e, loss_kind, S(stream));
+    if (loss) *loss = l;
+  });
+}

@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "assembly.hpp"
+#include "unet.hpp"
 
 namespace apx {
 
@@ -123,24 +124,43 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
   NewtonTest test;
   double res = res0;
   bool have_tangent = false, go = true;
+  bool learned = o.precond == APX_PRECOND_LEARNED;
   PcgRun<T> pcg(P, vals, inv, C.st);
+  DevBuf<double> b64, x64;
+  if (learned) {
+    APX_REQUIRE(P.net != nullptr, APX_ERR_STATE, "learned preconditioner requested but no network is attached");
+    b64.alloc(P.n_free);
+    x64.alloc(P.n_free);
+  }
   while (go) {
     if (!have_tangent || !o.reuse_tangent) {
-      C.tangent(vals, inv, prec, o.precond);
+      C.tangent(vals, inv, prec, learned ? APX_PRECOND_JACOBI : o.precond);
       have_tangent = true;
     }
-    gather_rhs(P, C.r, b, prec, C.st);
     double eta = o.lin_tol;
     if (o.forcing && res > 0.0) eta = std::max(eta, std::min(0.5, 0.5 * thr / res));
     auto t0 = std::chrono::steady_clock::now();
-    pcg.init(b, nullptr, eta, o.lin_atol, o.lin_maxiter, 0.0);
-    pcg.iterations = 0;
-    pcg.counted = 0;
-    KState hs = pcg.run();
-    pcg.collect_timing();
+    if (learned) {
+      gather_rhs(P, C.r, b64.p, APX_FP64, C.st);
+      FcgResult fr = fcg_solve<T>(P, *P.net, vals, P.net_g, b64.p, x64.p, eta, o.lin_atol, o.lin_maxiter, C.st);
+      s.krylov_iterations += fr.iterations;
+      if (fr.stagnated) {
+        // the network stopped reducing the residual: keep its progress, continue with Jacobi-PCG
+        s.fallbacks += 1;
+        learned = false;
+      }
+      scatter_update(P, C.u, x64.p, o.damping, APX_FP64, C.st);
+    } else {
+      gather_rhs(P, C.r, b, prec, C.st);
+      pcg.init(b, nullptr, eta, o.lin_atol, o.lin_maxiter, 0.0);
+      pcg.iterations = 0;
+      pcg.counted = 0;
+      KState hs = pcg.run();
+      pcg.collect_timing();
+      s.krylov_iterations += hs.k;
+      scatter_update(P, C.u, pcg.x, o.damping, prec, C.st);
+    }
     s.seconds_krylov += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    s.krylov_iterations += hs.k;
-    scatter_update(P, C.u, pcg.x, o.damping, prec, C.st);
     C.residual();
     res = C.norm();
     go = test.step(res, thr, o.maxiter, s);
@@ -196,13 +216,13 @@ static void newton_reliable(NewtonCtx& C, const double* dvals, const apx_newton_
 
 void newton(Plan& P, double* u, const double* dvals, const apx_newton_options& o, apx_newton_stats& s,
             hipStream_t st) {
-  APX_REQUIRE(o.precond == APX_PRECOND_JACOBI || o.precond == APX_PRECOND_NONE, APX_ERR_UNSUPPORTED,
-              "apx_newton: preconditioner not available in this entry point");
+  APX_REQUIRE(o.precond == APX_PRECOND_JACOBI || o.precond == APX_PRECOND_NONE || o.precond == APX_PRECOND_LEARNED,
+              APX_ERR_UNSUPPORTED, "apx_newton: unknown preconditioner");
   const auto t_start = std::chrono::steady_clock::now();
   if (P.r_full.n < (size_t)P.n_dofs) P.r_full.alloc(P.n_dofs);
   s = apx_newton_stats{};
   NewtonCtx C{P, u, P.r_full.p, st, {}, {}, s};
-  if (o.precision == APX_FP32 && o.reuse_tangent && o.reliable_delta > 0.0) {
+  if (o.precision == APX_FP32 && o.reuse_tangent && o.reliable_delta > 0.0 && o.precond != APX_PRECOND_LEARNED) {
     APX_REQUIRE(o.damping == 1.0, APX_ERR_ARG, "reliable-update refinement needs damping 1");
     newton_reliable(C, dvals, o);
   } else if (o.precision == APX_FP64) {

@@ -70,8 +70,11 @@ struct Plan {
   DevBuf<float> vals32, w32;
   DevBuf<double> r_full, diag_full;  // n_dofs
   DevBuf<double> partials;     // reduction partials
+  DevBuf<double> tw, tpart;    // network training workspace (4 n_free) and its reduction partials
   DevBuf<double> kstate;       // device Krylov scalars
   double* pinned = nullptr;    // host pinned scalars
+  struct UNet* net = nullptr;  // attached learned preconditioner (not owned)
+  const double* net_g = nullptr;  // nodal coefficient channel of the network input (device, optional)
   bool timing = false;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -1,0 +1,737 @@
+// Learned preconditioner runtime: U-Net construction, forward, backward,
+// Adam, and the conversions between reduced DOF vectors and the node grid.
+//
+// Architecture (levels l = 0..L-1, channels C_l, grids H_l = ceil(H_{l-1}/2)):
+//   a_0   = act(conv_in(x_0))
+//   d_l   = act(conv_e[l](a_l))            rd_l = R(d_l)
+//   a_l+1 = act(conv_down[l](rd_l))
+//   u_L-1 = act(conv_b(a_L-1))
+//   t_l   = conv_up[l](u_l+1)              s_l = d_l + P(t_l)
+//   u_l   = act(conv_d[l](s_l))
+//   y     = conv_out(u_0)                  (FP32)
+// R is full weighting and P bilinear interpolation between node-centred grids
+// (R = P^T, weights (1/2, 1, 1/2) per axis), so the network has the data flow
+// of a multigrid V-cycle with learned smoothing / transfer convolutions.
+// `act` is ReLU or the identity (a linear network, i.e. a fixed linear
+// preconditioner).  There is no counterpart in the reference.
+#include <cmath>
+#include <cstring>
+
+#include "assembly.hpp"
+#include "unet.hpp"
+#include "unet_kernels.hpp"
+
+namespace apx {
+using namespace nn;
+
+namespace {
+constexpr int kWgradBlocks = 512;
+
+int kblocks(int ci) { return ci >= 16 ? 9 * (ci / 16) : (9 * ci + 15) / 16; }
+
+// ---- conv dispatch ------------------------------------------------------------------
+template <int CI, int CO, bool F32>
+void launch_conv_t(const ConvArgs& a, hipStream_t st) {
+  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, 2048));
+  hipLaunchKernelGGL((k_conv3x3<CI, CO, F32>), dim3(g), dim3(kConvThreads), 0, st, a);
+}
+
+template <int CI, int CO>
+void launch_wgrad_t(const WgradArgs& a, hipStream_t st) {
+  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kWgradBlocks));
+  hipLaunchKernelGGL((k_conv3x3_wgrad<CI, CO>), dim3(g), dim3(kConvThreads), 0, st, a);
+}
+
+int wgrad_grid(const WgradArgs& a) {
+  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kWgradBlocks));
+}
+
+#define APX_CONV_CASES(X) \
+  X(1, 16) X(2, 16) X(16, 16) X(16, 32) X(32, 16) X(32, 32) X(32, 64) X(64, 32) X(64, 64) X(16, 64) X(64, 16)
+
+void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
+  if (f32) {
+    if (ci == 16 && co == 1) return launch_conv_t<16, 1, true>(a, st);
+    if (ci == 16 && co == 2) return launch_conv_t<16, 2, true>(a, st);
+    throw Error(APX_ERR_UNSUPPORTED, "FP32-output convolution only for the 16 -> 1|2 output layer");
+  }
+#define APX_X(A, B) \
+  if (ci == A && co == B) return launch_conv_t<A, B, false>(a, st);
+  APX_CONV_CASES(APX_X)
+#undef APX_X
+  if (ci == 1 && co == 16) return launch_conv_t<1, 16, false>(a, st);
+  if (ci == 2 && co == 16) return launch_conv_t<2, 16, false>(a, st);
+  throw Error(APX_ERR_UNSUPPORTED, "no convolution kernel for " + std::to_string(ci) + " -> " + std::to_string(co) +
+                                       " channels");
+}
+
+void launch_wgrad(int ci, int co, const WgradArgs& a, hipStream_t st) {
+#define APX_X(A, B) \
+  if (ci == A && co == B) return launch_wgrad_t<A, B>(a, st);
+  APX_CONV_CASES(APX_X)
+#undef APX_X
+  if (ci == 16 && co == 1) return launch_wgrad_t<16, 1>(a, st);
+  if (ci == 16 && co == 2) return launch_wgrad_t<16, 2>(a, st);
+  if (ci == 1 && co == 16) return launch_wgrad_t<1, 16>(a, st);
+  if (ci == 2 && co == 16) return launch_wgrad_t<2, 16>(a, st);
+  throw Error(APX_ERR_UNSUPPORTED, "no weight-gradient kernel for " + std::to_string(ci) + " -> " +
+                                       std::to_string(co) + " channels");
+}
+
+// ---- grid transfer ---------------------------------------------------------------------
+// P: fine(i, j) = sum_{I,J} w(i - 2I) w(j - 2J) coarse(I, J), w(0) = 1, w(+-1) = 1/2.
+// out = skip + P(coarse) (skip may alias out).
+__global__ void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip, uint16_t* out, int B, int H,
+                              int W, int Hc, int Wc, int C) {
+  const int64_t n = (int64_t)B * H * W * C;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t p = e / C;
+    const int j = (int)(p % W), i = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
+    const int I0 = i >> 1, J0 = j >> 1;
+    const int I1 = (i & 1) ? I0 + 1 : I0, J1 = (j & 1) ? J0 + 1 : J0;
+    const uint16_t* cb = coarse + (int64_t)b * Hc * Wc * C;
+    auto at = [&](int I, int J) { return (I < Hc && J < Wc) ? bf2f(cb[((int64_t)I * Wc + J) * C + c]) : 0.f; };
+    float v;
+    if (!(i & 1) && !(j & 1))
+      v = at(I0, J0);
+    else if (!(i & 1))
+      v = 0.5f * (at(I0, J0) + at(I0, J1));
+    else if (!(j & 1))
+      v = 0.5f * (at(I0, J0) + at(I1, J0));
+    else
+      v = 0.25f * (at(I0, J0) + at(I0, J1) + at(I1, J0) + at(I1, J1));
+    if (skip) v += bf2f(skip[e]);
+    out[e] = f2bf(v);
+  }
+}
+
+// R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj).
+__global__ void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int B, int H, int W, int Hc,
+                           int Wc, int C) {
+  const int64_t n = (int64_t)B * Hc * Wc * C;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t p = e / C;
+    const int J = (int)(p % Wc), I = (int)((p / Wc) % Hc), b = (int)(p / ((int64_t)Wc * Hc));
+    const uint16_t* fb = fine + (int64_t)b * H * W * C;
+    float v = 0.f;
+#pragma unroll
+    for (int di = -1; di <= 1; ++di)
+#pragma unroll
+      for (int dj = -1; dj <= 1; ++dj) {
+        const int i = 2 * I + di, j = 2 * J + dj;
+        if (i < 0 || i >= H || j < 0 || j >= W) continue;
+        const float w = (di ? 0.5f : 1.f) * (dj ? 0.5f : 1.f);
+        v += w * bf2f(fb[((int64_t)i * W + j) * C + c]);
+      }
+    out[e] = f2bf(v);
+  }
+}
+
+// ---- parameters ---------------------------------------------------------------------------
+__global__ void k_cast_weights(const float* theta, uint16_t* wf, int64_t w_off, int64_t wf_off, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    wf[wf_off + i] = f2bf(theta[w_off + i]);
+}
+
+// dgrad weights: wd[ci][t'*CO + co] = W[co][(8 - t')*CI + ci]
+__global__ void k_flip_weights(const float* theta, uint16_t* wd, int64_t w_off, int64_t wd_off, int CI, int CO,
+                               int wk, int d_cop, int d_wk) {
+  const int64_t n = (int64_t)d_cop * d_wk;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(i / d_wk), k = (int)(i % d_wk);
+    float v = 0.f;
+    if (ci < CI && k < 9 * CO) {
+      const int tp = k / CO, co = k % CO;
+      v = theta[w_off + (int64_t)co * wk + (8 - tp) * CI + ci];
+    }
+    wd[wd_off + i] = f2bf(v);
+  }
+}
+
+__global__ void k_reduce_wgrad(const float* partial, int nblk, int64_t stride, int cop, int wk, int co_real,
+                               float* grad, int64_t w_off, int64_t b_off) {
+  const int64_t n = (int64_t)cop * wk + co_real;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t src = i < (int64_t)cop * wk ? i : (int64_t)cop * wk + (i - (int64_t)cop * wk);
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * stride + src];
+    if (i < (int64_t)cop * wk)
+      grad[w_off + i] = s;
+    else
+      grad[b_off + (i - (int64_t)cop * wk)] = s;
+  }
+}
+
+__global__ void k_adam(float* theta, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    theta[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// ---- loss ---------------------------------------------------------------------------------------
+constexpr int kLossBlocks = 256;
+__global__ void k_loss_partials(const float* y, const float* t, int64_t per_img, int B, double* part) {
+  // part[(b * kLossBlocks + blk) * 2 + {0: sum (y-t)^2, 1: sum t^2}]
+  __shared__ double lds[4];
+  for (int b = 0; b < B; ++b) {
+    double e2 = 0.0, t2 = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < per_img; i += (int64_t)gridDim.x * blockDim.x) {
+      const double ti = t[(int64_t)b * per_img + i], di = (double)y[(int64_t)b * per_img + i] - ti;
+      e2 += di * di;
+      t2 += ti * ti;
+    }
+    e2 = block_sum<256>(e2, lds);
+    t2 = block_sum<256>(t2, lds);
+    if (threadIdx.x == 0) {
+      part[((int64_t)b * kLossBlocks + blockIdx.x) * 2 + 0] = e2;
+      part[((int64_t)b * kLossBlocks + blockIdx.x) * 2 + 1] = t2;
+    }
+  }
+}
+
+__global__ void k_loss_fin(double* part, int B, int nblk) {
+  // totals in part[B*kLossBlocks*2 + b*2 + {0,1}], loss in part[B*kLossBlocks*2 + 2B]
+  if (threadIdx.x != 0) return;
+  double loss = 0.0;
+  for (int b = 0; b < B; ++b) {
+    double e2 = 0.0, t2 = 0.0;
+    for (int k = 0; k < nblk; ++k) {
+      e2 += part[((int64_t)b * kLossBlocks + k) * 2 + 0];
+      t2 += part[((int64_t)b * kLossBlocks + k) * 2 + 1];
+    }
+    part[(int64_t)B * kLossBlocks * 2 + b * 2 + 0] = e2;
+    part[(int64_t)B * kLossBlocks * 2 + b * 2 + 1] = t2;
+    loss += e2 / fmax(t2, 1e-300);
+  }
+  part[(int64_t)B * kLossBlocks * 2 + 2 * B] = loss / B;
+}
+
+__global__ void k_loss_grad(const float* y, const float* t, int64_t per_img, int B, const double* part,
+                            uint16_t* gy) {
+  const int64_t n = per_img * B;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / per_img);
+    const double t2 = part[(int64_t)B * kLossBlocks * 2 + b * 2 + 1];
+    const double g = 2.0 * ((double)y[i] - (double)t[i]) / (fmax(t2, 1e-300) * B);
+    gy[i] = f2bf((float)g);
+  }
+}
+
+void grid1d(int64_t n, int& g) { g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+}  // namespace
+
+// ---- UNet -------------------------------------------------------------------------------------------
+void UNet::build(int L, const int* chans, int cin, int cout, int act, int B, int H0, int W0) {
+  APX_REQUIRE(L >= 1 && L <= kMaxLevels, APX_ERR_ARG, "levels must be 1..12");
+  APX_REQUIRE(cin == 1 || cin == 2, APX_ERR_UNSUPPORTED, "input channels must be 1 or 2");
+  APX_REQUIRE(cout == 1 || cout == 2, APX_ERR_UNSUPPORTED, "output channels must be 1 or 2");
+  APX_REQUIRE(B >= 1 && H0 >= 3 && W0 >= 3, APX_ERR_ARG, "invalid grid");
+  levels = L;
+  c_in = cin;
+  c_out = cout;
+  relu = act ? 1 : 0;
+  batch = B;
+  for (int l = 0; l < L; ++l) {
+    APX_REQUIRE(chans[l] == 16 || chans[l] == 32 || chans[l] == 64, APX_ERR_UNSUPPORTED,
+                "channels per level must be 16, 32 or 64");
+    ch[l] = chans[l];
+    H[l] = l ? (H[l - 1] + 1) / 2 : H0;
+    W[l] = l ? (W[l - 1] + 1) / 2 : W0;
+    APX_REQUIRE(H[l] >= 2 && W[l] >= 2, APX_ERR_ARG, "too many levels for the grid (coarsest level below 2 x 2)");
+  }
+  APX_REQUIRE(ch[0] == 16, APX_ERR_UNSUPPORTED, "the finest level must have 16 channels");
+  layers.clear();
+  auto add = [&](int ci, int co, int r, int f32) {
+    NetLayer ly;
+    ly.ci = ci;
+    ly.co = co;
+    ly.cop = round16(co);
+    ly.wk = kblocks(ci) * 16;
+    ly.relu = r;
+    ly.out_f32 = f32;
+    ly.d_cop = round16(ci);
+    ly.d_wk = kblocks(co) * 16;
+    layers.push_back(ly);
+    return (int)layers.size() - 1;
+  };
+  L_in = add(c_in, ch[0], relu, 0);
+  for (int l = 0; l + 1 < L; ++l) {
+    L_e[l] = add(ch[l], ch[l], relu, 0);
+    L_down[l] = add(ch[l], ch[l + 1], relu, 0);
+  }
+  L_b = add(ch[L - 1], ch[L - 1], relu, 0);
+  for (int l = L - 2; l >= 0; --l) {
+    L_up[l] = add(ch[l + 1], ch[l], 0, 0);
+    L_d[l] = add(ch[l], ch[l], relu, 0);
+  }
+  L_out = add(ch[0], c_out, 0, 1);
+  n_params = n_wf = n_wd = 0;
+  for (auto& ly : layers) {
+    ly.w_off = n_params;
+    n_params += (int64_t)ly.cop * ly.wk;
+    ly.b_off = n_params;
+    n_params += ly.co;
+    ly.wf_off = n_wf;
+    n_wf += (int64_t)ly.cop * ly.wk;
+    ly.wd_off = n_wd;
+    n_wd += (int64_t)ly.d_cop * ly.d_wk;
+  }
+  theta.alloc(n_params);
+  grad.alloc(n_params);
+  adam_m.alloc(n_params);
+  adam_v.alloc(n_params);
+  APX_HIP(hipMemset(theta.p, 0, theta.bytes()));
+  APX_HIP(hipMemset(grad.p, 0, grad.bytes()));
+  APX_HIP(hipMemset(adam_m.p, 0, adam_m.bytes()));
+  APX_HIP(hipMemset(adam_v.p, 0, adam_v.bytes()));
+  wf.alloc(n_wf);
+  wd.alloc(n_wd);
+  int64_t maxpart = 0;
+  for (auto& ly : layers) maxpart = std::max<int64_t>(maxpart, (int64_t)ly.cop * ly.wk + ly.cop);
+  wpart.alloc(maxpart * kWgradBlocks);
+  x0.alloc(pixels(0) * c_in);
+  for (int l = 0; l < L; ++l) {
+    a[l].alloc(pixels(l) * ch[l]);
+    u[l].alloc(pixels(l) * ch[l]);
+    gA[l].alloc(pixels(l) * ch[l]);
+    gU[l].alloc(pixels(l) * ch[l]);
+    if (l + 1 < L) {
+      d[l].alloc(pixels(l) * ch[l]);
+      s[l].alloc(pixels(l) * ch[l]);
+      gD[l].alloc(pixels(l) * ch[l]);
+      gS[l].alloc(pixels(l) * ch[l]);
+      rd[l].alloc(pixels(l + 1) * ch[l]);
+      t[l].alloc(pixels(l + 1) * ch[l]);
+      gR[l].alloc(pixels(l + 1) * ch[l]);
+      gT[l].alloc(pixels(l + 1) * ch[l]);
+    }
+  }
+  y.alloc(pixels(0) * c_out);
+  gy.alloc(pixels(0) * c_out);
+  loss_part.alloc((int64_t)batch * kLossBlocks * 2 + 2 * batch + 1);
+  scale.alloc(batch);
+  weights_ready = false;
+}
+
+void UNet::prepare_weights(hipStream_t st) {
+  for (auto& ly : layers) {
+    int g;
+    grid1d((int64_t)ly.cop * ly.wk, g);
+    hipLaunchKernelGGL(k_cast_weights, dim3(g), dim3(256), 0, st, theta.p, wf.p, ly.w_off, ly.wf_off,
+                       (int64_t)ly.cop * ly.wk);
+    grid1d((int64_t)ly.d_cop * ly.d_wk, g);
+    hipLaunchKernelGGL(k_flip_weights, dim3(g), dim3(256), 0, st, theta.p, wd.p, ly.w_off, ly.wd_off, ly.ci, ly.co,
+                       ly.wk, ly.d_cop, ly.d_wk);
+  }
+  APX_LAUNCH_CHECK();
+  weights_ready = true;
+}
+
+namespace {
+void conv_fwd(UNet& N, int li, int level, const uint16_t* x, void* y, hipStream_t st) {
+  const NetLayer& ly = N.layers[li];
+  ConvArgs a{};
+  a.x = x;
+  a.xmask = nullptr;
+  a.w = N.wf.p + ly.wf_off;
+  a.bias = N.theta.p + ly.b_off;
+  a.y = y;
+  a.yscale = nullptr;
+  a.B = N.batch;
+  a.H = N.H[level];
+  a.W = N.W[level];
+  a.relu = ly.relu;
+  launch_conv(ly.ci, ly.co, ly.out_f32 != 0, a, st);
+}
+
+// dX = conv^T(dY masked by relu(Y) > 0)
+void conv_dgrad(UNet& N, int li, int level, const uint16_t* dy, const uint16_t* ymask, uint16_t* dx, hipStream_t st) {
+  const NetLayer& ly = N.layers[li];
+  ConvArgs a{};
+  a.x = dy;
+  a.xmask = ly.relu ? ymask : nullptr;
+  a.w = N.wd.p + ly.wd_off;
+  a.bias = nullptr;
+  a.y = dx;
+  a.B = N.batch;
+  a.H = N.H[level];
+  a.W = N.W[level];
+  a.relu = 0;
+  launch_conv(ly.co, ly.ci, false, a, st);
+}
+
+void conv_wgrad(UNet& N, int li, int level, const uint16_t* x, const uint16_t* dy, const uint16_t* ymask,
+                hipStream_t st) {
+  const NetLayer& ly = N.layers[li];
+  WgradArgs a{};
+  a.x = x;
+  a.dy = dy;
+  a.ymask = ly.relu ? ymask : nullptr;
+  a.partial = N.wpart.p;
+  a.B = N.batch;
+  a.H = N.H[level];
+  a.W = N.W[level];
+  launch_wgrad(ly.ci, ly.co, a, st);
+  const int nblk = wgrad_grid(a);
+  int g;
+  grid1d((int64_t)ly.cop * ly.wk + ly.co, g);
+  hipLaunchKernelGGL(k_reduce_wgrad, dim3(g), dim3(256), 0, st, N.wpart.p, nblk, (int64_t)ly.cop * ly.wk + ly.cop,
+                     ly.cop, ly.wk, ly.co, N.grad.p, ly.w_off, ly.b_off);
+}
+
+void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const uint16_t* skip, uint16_t* out,
+                 hipStream_t st) {
+  int g;
+  grid1d(N.pixels(level_fine) * C, g);
+  hipLaunchKernelGGL(k_prolong_add, dim3(g), dim3(256), 0, st, coarse, skip, out, N.batch, N.H[level_fine],
+                     N.W[level_fine], N.H[level_fine + 1], N.W[level_fine + 1], C);
+}
+
+void restrict_(UNet& N, int level_fine, int C, const uint16_t* fine, uint16_t* out, hipStream_t st) {
+  int g;
+  grid1d(N.pixels(level_fine + 1) * C, g);
+  hipLaunchKernelGGL(k_restrict, dim3(g), dim3(256), 0, st, fine, out, N.batch, N.H[level_fine], N.W[level_fine],
+                     N.H[level_fine + 1], N.W[level_fine + 1], C);
+}
+}  // namespace
+
+void UNet::forward(hipStream_t st) {
+  APX_REQUIRE(weights_ready, APX_ERR_STATE, "network weights not prepared");
+  const int L = levels;
+  conv_fwd(*this, L_in, 0, x0.p, a[0].p, st);
+  for (int l = 0; l + 1 < L; ++l) {
+    conv_fwd(*this, L_e[l], l, a[l].p, d[l].p, st);
+    restrict_(*this, l, ch[l], d[l].p, rd[l].p, st);
+    conv_fwd(*this, L_down[l], l + 1, rd[l].p, a[l + 1].p, st);
+  }
+  conv_fwd(*this, L_b, L - 1, a[L - 1].p, u[L - 1].p, st);
+  for (int l = L - 2; l >= 0; --l) {
+    conv_fwd(*this, L_up[l], l + 1, u[l + 1].p, t[l].p, st);
+    prolong_add(*this, l, ch[l], t[l].p, d[l].p, s[l].p, st);
+    conv_fwd(*this, L_d[l], l, s[l].p, u[l].p, st);
+  }
+  conv_fwd(*this, L_out, 0, u[0].p, y.p, st);
+  APX_LAUNCH_CHECK();
+}
+
+double UNet::loss_backward(const float* target, hipStream_t st) {
+  const int L = levels;
+  const int64_t per_img = (int64_t)H[0] * W[0] * c_out;
+  hipLaunchKernelGGL(k_loss_partials, dim3(kLossBlocks), dim3(256), 0, st, y.p, target, per_img, batch, loss_part.p);
+  hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, st, loss_part.p, batch, kLossBlocks);
+  int g;
+  grid1d(per_img * batch, g);
+  hipLaunchKernelGGL(k_loss_grad, dim3(g), dim3(256), 0, st, y.p, target, per_img, batch, loss_part.p, gy.p);
+  backward(st);
+  double loss = 0.0;
+  APX_HIP(hipMemcpyAsync(&loss, loss_part.p + (int64_t)batch * kLossBlocks * 2 + 2 * batch, sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+  APX_HIP(hipStreamSynchronize(st));
+  return loss;
+}
+
+void UNet::backward(hipStream_t st) {
+  const int L = levels;
+  // backward through the decoder
+  conv_wgrad(*this, L_out, 0, u[0].p, gy.p, nullptr, st);
+  conv_dgrad(*this, L_out, 0, gy.p, nullptr, gU[0].p, st);
+  for (int l = 0; l + 1 < L; ++l) {
+    conv_wgrad(*this, L_d[l], l, s[l].p, gU[l].p, u[l].p, st);
+    conv_dgrad(*this, L_d[l], l, gU[l].p, u[l].p, gS[l].p, st);
+    restrict_(*this, l, ch[l], gS[l].p, gT[l].p, st);  // P^T = R
+    conv_wgrad(*this, L_up[l], l + 1, u[l + 1].p, gT[l].p, nullptr, st);
+    conv_dgrad(*this, L_up[l], l + 1, gT[l].p, nullptr, gU[l + 1].p, st);
+  }
+  conv_wgrad(*this, L_b, L - 1, a[L - 1].p, gU[L - 1].p, u[L - 1].p, st);
+  conv_dgrad(*this, L_b, L - 1, gU[L - 1].p, u[L - 1].p, gA[L - 1].p, st);
+  for (int l = L - 2; l >= 0; --l) {
+    conv_wgrad(*this, L_down[l], l + 1, rd[l].p, gA[l + 1].p, a[l + 1].p, st);
+    conv_dgrad(*this, L_down[l], l + 1, gA[l + 1].p, a[l + 1].p, gR[l].p, st);
+    prolong_add(*this, l, ch[l], gR[l].p, gS[l].p, gD[l].p, st);  // R^T = P, plus the skip
+    conv_wgrad(*this, L_e[l], l, a[l].p, gD[l].p, d[l].p, st);
+    conv_dgrad(*this, L_e[l], l, gD[l].p, d[l].p, gA[l].p, st);
+  }
+  conv_wgrad(*this, L_in, 0, x0.p, gA[0].p, a[0].p, st);
+  if (!relu) {
+    // linear network: no biases, so the preconditioner is linear (and odd)
+    for (auto& ly : layers) APX_HIP(hipMemsetAsync(grad.p + ly.b_off, 0, ly.co * sizeof(float), st));
+  }
+  APX_LAUNCH_CHECK();
+}
+
+void UNet::adam(float lr, float b1, float b2, float eps, int step, hipStream_t st) {
+  int g;
+  grid1d(n_params, g);
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(g), dim3(256), 0, st, theta.p, grad.p, adam_m.p, adam_v.p, n_params, lr, b1, b2, eps,
+                     c1, c2);
+  APX_LAUNCH_CHECK();
+  prepare_weights(st);
+}
+
+// ---- reduced vector <-> node grid ----------------------------------------------------------------------
+namespace {
+template <class T>
+__global__ void k_sq_partials(int64_t n, const T* r, double* part) {
+  __shared__ double lds[4];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = (double)r[i];
+    s += v * v;
+  }
+  s = block_sum<256>(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_scale_fin(const double* part, int nb, const double* rr_src, int64_t n, float* scale, int img) {
+  __shared__ double lds[4];
+  double s = 0.0;
+  if (rr_src) {
+    s = *rr_src;
+  } else {
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+    s = block_sum<256>(s, lds);
+  }
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(s / (double)(n > 0 ? n : 1));
+    scale[img] = rms > 0.0 ? (float)rms : 1.f;
+  }
+}
+
+template <class T>
+__global__ void k_to_grid(int64_t n_nodes, int d, int cin, const uint8_t* dmask, const int32_t* full2red, const T* r,
+                          const double* g, const float* scale, int img, uint16_t* x) {
+  const float inv = 1.f / scale[img];
+  for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < n_nodes;
+       node += (int64_t)gridDim.x * blockDim.x) {
+    uint16_t* xo = x + ((int64_t)img * n_nodes + node) * cin;
+    for (int c = 0; c < d; ++c) {
+      const int64_t gd = node * d + c;
+      xo[c] = dmask[gd] ? (uint16_t)0 : f2bf((float)r[full2red[gd]] * inv);
+    }
+    if (cin > d) xo[d] = f2bf(g ? (float)g[node] : 0.f);
+  }
+}
+
+template <class T>
+__global__ void k_from_grid(int64_t n_free, int d, const int32_t* red2full, const float* y, const float* scale,
+                            int img, int64_t n_nodes, T* z) {
+  const float s = scale[img];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_free; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t gd = red2full[i];
+    z[i] = (T)(s * y[(int64_t)img * n_nodes * d + gd]);
+  }
+}
+}  // namespace
+
+void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, const double* g_nodal,
+                                const double* rr_src, int img, hipStream_t st) {
+  APX_REQUIRE((int64_t)N.H[0] * N.W[0] == P.n_nodes, APX_ERR_ARG, "network grid does not match the plan's nodes");
+  APX_REQUIRE(N.c_in == P.d || N.c_in == P.d + 1, APX_ERR_ARG, "network input channels do not match the DOFs");
+  APX_REQUIRE(img >= 0 && img < N.batch, APX_ERR_ARG, "image index out of range");
+  ensure_krylov_scratch(P, st);
+  int nb = 1;
+  if (!rr_src) {
+    grid1d(P.n_free, nb);
+    nb = std::min(nb, 1024);
+    if (r_f64)
+      hipLaunchKernelGGL(k_sq_partials<double>, dim3(nb), dim3(256), 0, st, P.n_free, (const double*)r, P.partials.p);
+    else
+      hipLaunchKernelGGL(k_sq_partials<float>, dim3(nb), dim3(256), 0, st, P.n_free, (const float*)r, P.partials.p);
+  }
+  hipLaunchKernelGGL(k_scale_fin, dim3(1), dim3(256), 0, st, P.partials.p, nb, rr_src, P.n_free, N.scale.p, img);
+  int g;
+  grid1d(P.n_nodes, g);
+  if (r_f64)
+    hipLaunchKernelGGL(k_to_grid<double>, dim3(g), dim3(256), 0, st, P.n_nodes, P.d, N.c_in, P.dmask.p, P.full2red.p,
+                       (const double*)r, g_nodal, N.scale.p, img, N.x0.p);
+  else
+    hipLaunchKernelGGL(k_to_grid<float>, dim3(g), dim3(256), 0, st, P.n_nodes, P.d, N.c_in, P.dmask.p, P.full2red.p,
+                       (const float*)r, g_nodal, N.scale.p, img, N.x0.p);
+  APX_LAUNCH_CHECK();
+}
+
+namespace {
+__global__ void k_target_grid(int64_t n_dofs, const uint8_t* dmask, const int32_t* full2red, const double* e,
+                              const float* scale, int img, float* t) {
+  const float inv = 1.f / scale[img];
+  for (int64_t gd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gd < n_dofs; gd += (int64_t)gridDim.x * blockDim.x)
+    t[(int64_t)img * n_dofs + gd] = dmask[gd] ? 0.f : (float)e[full2red[gd]] * inv;
+}
+}  // namespace
+
+void net_target_from_reduced_img(Plan& P, UNet& N, const double* e, float* target, int img, hipStream_t st) {
+  APX_REQUIRE(N.c_out == P.d, APX_ERR_ARG, "network output channels do not match the DOFs");
+  APX_REQUIRE(img >= 0 && img < N.batch, APX_ERR_ARG, "image index out of range");
+  int g;
+  grid1d(P.n_dofs, g);
+  hipLaunchKernelGGL(k_target_grid, dim3(g), dim3(256), 0, st, P.n_dofs, P.dmask.p, P.full2red.p, e, N.scale.p, img,
+                     target);
+  APX_LAUNCH_CHECK();
+}
+
+void net_output_to_reduced_img(Plan& P, UNet& N, void* z, int z_f64, int img, hipStream_t st) {
+  APX_REQUIRE(N.c_out == P.d, APX_ERR_ARG, "network output channels do not match the DOFs");
+  int g;
+  grid1d(P.n_free, g);
+  if (z_f64)
+    hipLaunchKernelGGL(k_from_grid<double>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, N.scale.p,
+                       img, P.n_nodes, (double*)z);
+  else
+    hipLaunchKernelGGL(k_from_grid<float>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, N.scale.p,
+                       img, P.n_nodes, (float*)z);
+  APX_LAUNCH_CHECK();
+}
+
+// ---- residual-loss training step -------------------------------------------------------------
+// For image b with residual r_b (reduced, FP64) and optional error e_b = A^-1 r_b:
+//   z_b = s_b y_b on the free DOFs,  L_b = ||r_b - A z_b||^2 / ||r_b||^2 + w_e ||z_b - e_b||^2 / ||e_b||^2
+// (loss = mean_b L_b).  The residual term weights every eigenmode of A alike
+// (|lambda mu - 1|^2 for network gain mu), which is what a preconditioner
+// needs; the error term alone would fit only the smooth modes.
+namespace {
+constexpr int kRedBlocks = 512;
+
+__global__ void k_res_terms(int64_t n, const double* r, const double* w, const double* z, const double* e,
+                            double* res, double* part) {
+  __shared__ double lds[4];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double ri = r[i], di = ri - w[i];
+    res[i] = di;
+    s[0] += di * di;
+    s[1] += ri * ri;
+    if (e) {
+      const double de = z[i] - e[i];
+      s[2] += de * de;
+      s[3] += e[i] * e[i];
+    }
+  }
+  for (int j = 0; j < 4; ++j) {
+    const double t = block_sum<256>(s[j], lds);
+    if (threadIdx.x == 0) part[j * kRedBlocks + blockIdx.x] = t;
+  }
+}
+
+// part[j * kRedBlocks + block] = sum a_j . b_j (j = 0..2; a null a_j gives 0)
+__global__ void k_dots3(int64_t n, const double* a0, const double* b0, const double* a1, const double* b1,
+                        const double* a2, const double* b2, double* part) {
+  __shared__ double lds[4];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    s0 += a0[i] * b0[i];
+    if (a1) s1 += a1[i] * b1[i];
+    if (a2) s2 += a2[i] * b2[i];
+  }
+  s0 = block_sum<256>(s0, lds);
+  s1 = block_sum<256>(s1, lds);
+  s2 = block_sum<256>(s2, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s0;
+    part[kRedBlocks + blockIdx.x] = s1;
+    part[2 * kRedBlocks + blockIdx.x] = s2;
+  }
+}
+
+__global__ void k_grad_out(int64_t n, const int32_t* red2full, const double* Ares, const double* z, const double* e,
+                           double cr, double ce, const float* scale, int img, float inv_batch, int64_t n_dofs,
+                           uint16_t* gy) {
+  const double s = scale[img];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double g = -cr * Ares[i];
+    if (e) g += ce * (z[i] - e[i]);
+    gy[(int64_t)img * n_dofs + red2full[i]] = f2bf((float)(s * g * inv_batch));
+  }
+}
+}  // namespace
+
+double unet_residual_loss_backward(Plan& P, UNet& N, const double* vals_sell, const double* const* r,
+                                   const double* const* e, double w_e, int mode, hipStream_t st) {
+  const int64_t n = P.n_free;
+  if (P.tw.n < (size_t)(4 * n)) P.tw.alloc(4 * n);
+  double *z = P.tw.p, *w = z + n, *res = z + 2 * n, *Ares = z + 3 * n;
+  if (P.tpart.n < (size_t)(4 * kRedBlocks)) P.tpart.alloc(4 * kRedBlocks);
+  std::vector<double> hp(4 * kRedBlocks);
+  int g;
+  grid1d(n, g);
+  const int gr = std::min(g, kRedBlocks);
+  APX_HIP(hipMemsetAsync(N.gy.p, 0, N.gy.bytes(), st));
+  // sign of the (definite) matrix from its first diagonal entry
+  double d0 = 1.0;
+  {
+    int32_t slot0 = 0;
+    APX_HIP(hipMemcpy(&slot0, P.diag_slot.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    APX_HIP(hipMemcpy(&d0, vals_sell + slot0, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  const double sA = d0 < 0.0 ? -1.0 : 1.0;
+  std::vector<float> hscale(N.batch);
+  APX_HIP(hipMemcpy(hscale.data(), N.scale.p, N.batch * sizeof(float), hipMemcpyDeviceToHost));
+  double loss = 0.0;
+  for (int b = 0; b < N.batch; ++b) {
+    net_output_to_reduced_img(P, N, z, 1, b, st);
+    spmv(P, vals_sell, z, w, APX_FP64, st);
+    hipLaunchKernelGGL(k_res_terms, dim3(gr), dim3(256), 0, st, n, r[b], w, z, e ? e[b] : nullptr, res,
+                       P.tpart.p);
+    APX_HIP(hipMemcpyAsync(hp.data(), P.tpart.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    APX_HIP(hipStreamSynchronize(st));
+    double t[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < gr; ++k) t[j] += hp[j * kRedBlocks + k];
+    const double rr = std::max(t[1], 1e-300), ee = std::max(t[3], 1e-300);
+    const double* eb = (e && e[b]) ? e[b] : nullptr;
+    if (mode == 1 || mode == 2) {
+      double d[3] = {0.0, 0.0, 0.0};
+      auto dots = [&](const double* a0, const double* b0, const double* a1, const double* b1, const double* a2,
+                      const double* b2) {
+        hipLaunchKernelGGL(k_dots3, dim3(gr), dim3(256), 0, st, n, a0, b0, a1, b1, a2, b2, P.tpart.p);
+        APX_HIP(hipMemcpyAsync(hp.data(), P.tpart.p, 3 * kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+        APX_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < gr; ++k) d[j] += hp[j * kRedBlocks + k];
+      };
+      double cr;
+      if (mode == 1) {
+        // energy functional J(z) = sA (z.Az / 2 - z.r) / (s^2 n), gradient sA (A z - r) / (s^2 n)
+        const double sb = hscale[b], norm = sb * sb * (double)n;
+        dots(z, w, z, r[b], nullptr, nullptr);
+        loss += sA * (0.5 * d[0] - d[1]) / norm + (eb ? w_e * t[2] / ee : 0.0);
+        cr = sA / norm;
+      } else {
+        // relative A-norm error (z - e).Ahat(z - e) / e.Ahat e, Ahat = sA A SPD, A e = r:
+        // Ahat (z - e) = -sA res, so the value is -sA (z - e).res / E and the gradient -2 sA res / E
+        APX_REQUIRE(eb != nullptr, APX_ERR_ARG, "the A-norm loss needs the exact errors e");
+        dots(z, res, eb, res, eb, r[b]);
+        const double E = std::max(sA * d[2], 1e-300);
+        loss += -sA * (d[0] - d[1]) / E + w_e * t[2] / ee;
+        cr = 2.0 * sA / E;
+      }
+      // g = -cr res (+ the error term) through the Ares slot of k_grad_out
+      hipLaunchKernelGGL(k_grad_out, dim3(g), dim3(256), 0, st, n, P.red2full.p, res, z, eb, cr, 2.0 * w_e / ee,
+                         N.scale.p, b, 1.f / N.batch, P.n_dofs, N.gy.p);
+    } else {
+      loss += t[0] / rr + (eb ? w_e * t[2] / ee : 0.0);
+      spmv(P, vals_sell, res, Ares, APX_FP64, st);
+      // d/dz: -2 A (r - A z) / ||r||^2 + 2 w_e (z - e) / ||e||^2 ; d/dy = s d/dz
+      hipLaunchKernelGGL(k_grad_out, dim3(g), dim3(256), 0, st, n, P.red2full.p, Ares, z, eb, 2.0 / rr,
+                         2.0 * w_e / ee, N.scale.p, b, 1.f / N.batch, P.n_dofs, N.gy.p);
+    }
+    APX_LAUNCH_CHECK();
+  }
+  N.backward(st);
+  APX_HIP(hipStreamSynchronize(st));
+  return loss / N.batch;
+}
+
+}  // namespace apx

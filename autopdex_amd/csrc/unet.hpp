@@ -1,0 +1,79 @@
+// Learned preconditioner: multigrid-structured U-Net on the structured node grid.
+#pragma once
+#include "plan.hpp"
+
+namespace apx {
+
+constexpr int kMaxLevels = 12;
+
+struct NetLayer {
+  int ci = 0, co = 0;      // stored channel counts
+  int cop = 0, wk = 0;     // padded output channels, packed K length
+  int relu = 0;
+  int out_f32 = 0;
+  int64_t w_off = 0, b_off = 0;  // offsets into the FP32 parameter vector
+  int64_t wf_off = 0;            // offset into bf16 forward weights [cop][wk]
+  int64_t wd_off = 0;            // offset into bf16 backward-data weights
+  int d_cop = 0, d_wk = 0;       // dgrad conv: ci' = co, co' = ci
+};
+
+struct UNet {
+  int levels = 0, c_in = 0, c_out = 0, relu = 0, batch = 1;
+  int ch[kMaxLevels] = {};
+  int H[kMaxLevels] = {}, W[kMaxLevels] = {};
+  std::vector<NetLayer> layers;
+  // layer indices
+  int L_in = -1, L_b = -1, L_out = -1;
+  int L_e[kMaxLevels], L_down[kMaxLevels], L_up[kMaxLevels], L_d[kMaxLevels];
+  int64_t n_params = 0, n_wf = 0, n_wd = 0;
+  DevBuf<float> theta, grad, adam_m, adam_v, wpart;
+  DevBuf<uint16_t> wf, wd;
+  // activations (bf16): x0; per level a, d, u, s, t, rd
+  DevBuf<uint16_t> x0;
+  DevBuf<uint16_t> a[kMaxLevels], d[kMaxLevels], u[kMaxLevels], s[kMaxLevels], t[kMaxLevels], rd[kMaxLevels];
+  DevBuf<float> y;                 // output (FP32), B x H x W x c_out
+  // gradient buffers (bf16)
+  DevBuf<uint16_t> gy, gA[kMaxLevels], gD[kMaxLevels], gU[kMaxLevels], gS[kMaxLevels], gT[kMaxLevels],
+      gR[kMaxLevels];
+  DevBuf<double> loss_part;
+  DevBuf<float> scale;             // per-image input scale
+  bool weights_ready = false;
+
+  void build(int levels, const int* channels, int c_in, int c_out, int relu, int batch, int H0, int W0);
+  void prepare_weights(hipStream_t st);   // FP32 master -> bf16 forward / backward-data layouts
+  void forward(hipStream_t st);           // x0 -> y
+  // loss = mean_b ||y_b - t_b||^2 / ||t_b||^2 ; fills grad (FP32) via backward
+  double loss_backward(const float* target, hipStream_t st);
+  // backward pass from the output gradient gy (bf16) into grad
+  void backward(hipStream_t st);
+  void adam(float lr, float b1, float b2, float eps, int step, hipStream_t st);
+  int64_t pixels(int l) const { return (int64_t)batch * H[l] * W[l]; }
+};
+
+// Preconditioner use: z = s * net(r / s) on the plan's free DOFs (grid = nodes).
+// rr_src (device, optional) holds sum r^2 over the free DOFs; s = sqrt(rr / n_free).
+void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, const double* g_nodal,
+                                const double* rr_src, int img, hipStream_t st);
+void net_output_to_reduced_img(Plan& P, UNet& N, void* z, int z_f64, int img, hipStream_t st);
+// target t = e / s of image img (grid layout, FP32) for training
+void net_target_from_reduced_img(Plan& P, UNet& N, const double* e, float* target, int img, hipStream_t st);
+
+// Residual-loss backward for the images already run forward (inputs from
+// residuals r[b]); vals_sell: FP64 matrix in the plan's SELL layout.
+// mode 0: residual loss ||r - A z||^2 / ||r||^2; mode 1: energy functional
+// sA (z.Az/2 - z.r) / (s^2 n) with sA the sign of A (minimizer z = A^-1 r);
+// mode 2: relative A-norm error (z-e).sA A (z-e) / e.sA A e (needs e = A^-1 r).
+double unet_residual_loss_backward(Plan& P, UNet& N, const double* vals_sell, const double* const* r,
+                                   const double* const* e, double w_e, int mode, hipStream_t st);
+
+struct FcgResult {
+  int iterations = 0;
+  bool converged = false, stagnated = false;
+  double rr0 = 0.0, rr = 0.0;
+};
+// Flexible CG with the network preconditioner on the plan's SELL matrix (values T).
+template <class T>
+FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, const double* b, double* x, double tol,
+                    double atol, int maxiter, hipStream_t st);
+
+}  // namespace apx

@@ -1,0 +1,240 @@
+"""Learned preconditioner: a multigrid-structured U-Net trained on the GPU.
+
+The network approximates A^-1 r on the structured node grid and is used
+inside flexible CG (static_settings['type of preconditioner'] = 'learned').
+Training data are (residual, error) pairs that the plain solver itself
+produces: Jacobi-PCG iterates x_k of solves A x = b give r_k = b - A x_k and
+e_k = x* - x_k.  Forward, backward and Adam run in libautopdex_amd's HIP
+kernels (bf16 MFMA convolutions, FP32 master weights).
+
+No counterpart in the reference; it plugs into the reference's Newton /
+linear-solve flow through the 'hip' backend's preconditioner option.
+"""
+import ctypes as C
+import hashlib
+import os
+import time
+
+import numpy as np
+
+from . import _native as N
+from .backend import current_stream, torch_cuda
+
+def _call(name, *args):
+    N.call(name, *args)
+
+
+def _lib():
+    return N.lib()
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class UNetPreconditioner:
+    """Network on an (height x width) node grid; levels with channels[l] in {16, 32, 64}."""
+
+    def __init__(self, height, width, levels=4, channels=(16, 32, 64, 64), c_in=1, c_out=1, relu=False, batch=1,
+                 seed=0):
+        torch_cuda()
+        self.height, self.width, self.levels = int(height), int(width), int(levels)
+        self.channels = tuple(int(c) for c in channels)[:levels]
+        self.c_in, self.c_out, self.relu, self.batch = int(c_in), int(c_out), bool(relu), int(batch)
+        ch = (C.c_int * len(self.channels))(*self.channels)
+        h = C.c_void_p()
+        _call("apx_unet_create", C.byref(h), self.levels, ch, self.c_in, self.c_out, int(self.relu), self.batch,
+              self.height, self.width)
+        self.handle = h
+        n, nl = C.c_int64(), C.c_int()
+        _call("apx_unet_num_params", h, C.byref(n), C.byref(nl))
+        self.n_params, self.n_layers = n.value, nl.value
+        self.layers = []
+        for i in range(self.n_layers):
+            w, b = C.c_int64(), C.c_int64()
+            ci, co, cop, wk, r, lv = (C.c_int() for _ in range(6))
+            _call("apx_unet_layer", h, i, C.byref(w), C.byref(b), C.byref(ci), C.byref(co), C.byref(cop),
+                  C.byref(wk), C.byref(r), C.byref(lv))
+            self.layers.append(dict(w_off=w.value, b_off=b.value, ci=ci.value, co=co.value, co_pad=cop.value,
+                                    k_pad=wk.value, relu=r.value, level=lv.value))
+        self.set_params(self.initial_params(seed))
+        self.step = 0
+
+    def initial_params(self, seed):
+        """He-normal weights: std sqrt(gain / (9 ci)), gain 2 with ReLU else 1; zero biases.  The
+        output layer is scaled down so the untrained network starts close to zero."""
+        rng = np.random.default_rng(seed)
+        theta = np.zeros(self.n_params, dtype=np.float32)
+        for i, ly in enumerate(self.layers):
+            std = np.sqrt((2.0 if ly["relu"] else 1.0) / (9 * ly["ci"]))
+            if i == self.n_layers - 1:
+                std *= 0.1
+            W = np.zeros((ly["co_pad"], ly["k_pad"]), dtype=np.float32)
+            W[:ly["co"], :9 * ly["ci"]] = rng.standard_normal((ly["co"], 9 * ly["ci"])) * std
+            theta[ly["w_off"]:ly["w_off"] + W.size] = W.ravel()
+        return theta
+
+    def set_params(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        assert theta.size == self.n_params
+        _call("apx_unet_set_params", self.handle, theta.ctypes.data)
+
+    def get_params(self):
+        out = np.empty(self.n_params, dtype=np.float32)
+        _call("apx_unet_get_params", self.handle, out.ctypes.data)
+        return out
+
+    def get_grads(self):
+        out = np.empty(self.n_params, dtype=np.float32)
+        _call("apx_unet_get_grads", self.handle, out.ctypes.data)
+        return out
+
+    def buffers(self):
+        x, y, s = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _call("apx_unet_buffers", self.handle, C.byref(x), C.byref(y), C.byref(s))
+        return x.value, y.value, s.value
+
+    def forward(self):
+        _call("apx_unet_forward", self.handle, current_stream())
+
+    def loss_backward(self, target):
+        lval = C.c_double()
+        _call("apx_unet_loss_backward", self.handle, _vp(target), C.byref(lval), current_stream())
+        return lval.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step += 1
+        _call("apx_unet_adam", self.handle, lr, beta1, beta2, eps, self.step, current_stream())
+
+    def load_pair(self, problem, r, e=None, target=None, g=None, image=0):
+        _call("apx_unet_load_pair", self.handle, problem.plan, _vp(r), _vp(g), _vp(e), _vp(target), image,
+              current_stream())
+
+    def apply(self, problem, r, g=None):
+        z = r.new_empty(r.shape)
+        _call("apx_unet_apply", self.handle, problem.plan, _vp(r), _vp(g), _vp(z), current_stream())
+        return z
+
+    def attach(self, problem, g=None):
+        """Use this network as the plan's preconditioner; g = nodal input channel (device FP64)."""
+        _call("apx_plan_attach_unet", problem.plan, self.handle, _vp(g))
+        problem._attached_net = (self, g)
+
+    def spec(self):
+        return dict(height=self.height, width=self.width, levels=self.levels, channels=list(self.channels),
+                    c_in=self.c_in, c_out=self.c_out, relu=self.relu)
+
+    def save(self, path):
+        theta = self.get_params()
+        digest = hashlib.sha256(theta.tobytes()).hexdigest()
+        np.savez(path, theta=theta, spec=np.array(repr(self.spec())), sha256=np.array(digest))
+        return digest
+
+    @classmethod
+    def load(cls, path, height=None, width=None, batch=1):
+        d = np.load(path, allow_pickle=False)
+        import ast
+        spec = ast.literal_eval(str(d["spec"]))
+        theta = d["theta"]
+        if hashlib.sha256(theta.tobytes()).hexdigest() != str(d["sha256"]):
+            raise ValueError(f"{path}: parameter hash mismatch")
+        net = cls(height or spec["height"], width or spec["width"], spec["levels"], spec["channels"], spec["c_in"],
+                  spec["c_out"], spec["relu"], batch=batch)
+        net.set_params(theta)
+        return net
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib().apx_unet_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def fcg(problem, vals, b, tol=1e-10, atol=0.0, maxiter=2000, precision="fp64"):
+    """One flexible-CG solve with the attached network (x0 = 0); vals in CSR order."""
+    x = b.new_empty(b.shape)
+    ks = N.KrylovStats()
+    p = N.APX_FP64 if precision == "fp64" else N.APX_FP32
+    _call("apx_fcg", problem.plan, _vp(vals), _vp(b), _vp(x), p, tol, atol, maxiter, C.byref(ks), current_stream())
+    return x, ks
+
+
+def pcg(problem, vals, b, tol=1e-10, maxiter=0, precision="fp64"):
+    x = b.new_empty(b.shape)
+    ks = N.KrylovStats()
+    p = N.APX_FP64 if precision == "fp64" else N.APX_FP32
+    N.call("apx_pcg", problem.plan, _vp(vals), _vp(b), _vp(x), p, N.APX_PRECOND_JACOBI, tol, 0.0, maxiter,
+           C.byref(ks), current_stream())
+    return x, ks
+
+
+def spmv(problem, vals, x):
+    y = x.new_empty(x.shape)
+    N.call("apx_spmv", problem.plan, _vp(vals), _vp(x), _vp(y), N.APX_FP64, current_stream())
+    return y
+
+
+def tangent(problem, u=None):
+    torch = torch_cuda()
+    if u is None:
+        u = torch.zeros(problem.n_dofs, dtype=torch.float64, device=problem.device)
+    vals = torch.empty(problem.nnz, dtype=torch.float64, device=problem.device)
+    N.call("apx_assemble_tangent", problem.plan, _vp(u), _vp(vals), N.APX_FP64, current_stream())
+    return vals
+
+
+def solver_pairs(problem, vals, rhs, checkpoints=(0, 4, 16, 48, 128, 320, 800, 2000), tol=1e-10):
+    """(r_k, e_k) pairs from Jacobi-PCG iterates of A x = b: e_k = x* - x_k, r_k = b - A x_k.
+    Restarted PCG runs with maxiter = k reproduce the iterates x_k of the full solve."""
+    torch = torch_cuda()
+    xs, ks = pcg(problem, vals, rhs, tol=tol)
+    out = []
+    for k in checkpoints:
+        if k >= ks.iterations:
+            break
+        if k == 0:
+            xk = torch.zeros_like(rhs)
+        else:
+            xk, _ = pcg(problem, vals, rhs, tol=0.0, maxiter=k)
+        out.append((rhs - spmv(problem, vals, xk), xs - xk))
+    return out
+
+
+_LOSS_KINDS = {"residual": 0, "energy": 1, "anorm": 2}
+
+
+def train(net, problem, pairs, vals, steps=500, lr=2e-3, lr_final=2e-4, w_e=0.0, g=None, seed=0, log_every=50,
+          log=print, time_budget=None, warmup=0, loss="anorm"):
+    """Adam over random batches of (r, e[, g]) pairs; vals: FP64 tangent (CSR).  loss="anorm":
+    relative A-norm error (z-e).|A|(z-e) / e.|A|e of z = M(r), e = A^-1 r (the quantity CG
+    minimizes); "energy": sA (z.Az/2 - z.r) / r.r without e; "residual": ||r - A z||^2 / ||r||^2;
+    optionally + w_e x relative error ||z - e||^2 / ||e||^2.
+    Learning rate: linear warm-up over `warmup` steps, then geometric decay from lr to lr_final."""
+    torch_cuda()
+    loss_kind = loss
+    if loss_kind not in _LOSS_KINDS:
+        raise ValueError(f"loss must be one of {sorted(_LOSS_KINDS)}")
+    rng = np.random.default_rng(seed)
+    B = net.batch
+    history = []
+    t0 = time.perf_counter()
+    for it in range(steps):
+        idx = rng.integers(0, len(pairs), size=B)
+        r_list = (C.c_void_p * B)(*[pairs[i][0].data_ptr() for i in idx])
+        e_list = ((C.c_void_p * B)(*[pairs[i][1].data_ptr() for i in idx])
+                  if (w_e > 0 or loss_kind == "anorm") else None)
+        gs = [(pairs[i][2] if len(pairs[i]) > 2 else g) for i in idx]
+        g_list = (C.c_void_p * B)(*[x.data_ptr() for x in gs]) if gs[0] is not None else None
+        lval = C.c_double()
+        _call("apx_unet_train_step", net.handle, problem.plan, _vp(vals), r_list, g_list, e_list,
+              _LOSS_KINDS[loss_kind], float(w_e), C.byref(lval), current_stream())
+        frac = it / max(steps - 1, 1)
+        warm = min(1.0, (it + 1) / max(1, warmup))
+        net.adam(warm * lr * (lr_final / lr) ** frac)
+        history.append(lval.value)
+        if log and (it % log_every == 0 or it == steps - 1):
+            log(f"[train] step {it} loss {lval.value:.4e} ({time.perf_counter() - t0:.1f} s)")
+        if time_budget and time.perf_counter() - t0 > time_budget:
+            break
+    return history

@@ -168,6 +168,63 @@ typedef struct apx_newton_stats {
 int apx_newton(apx_plan* plan, double* u_dev, const double* dirichlet_values_dev,
                const apx_newton_options* opts, apx_newton_stats* stats, void* stream);
 
+/* ---- learned preconditioner (no counterpart in the reference) ---------------
+ * A multigrid-structured U-Net on the structured node grid (height x width =
+ * the plan's nodes, node index = row * width + col as produced by
+ * mesher.structured_mesh).  Levels l = 0..levels-1 with channels[l] in
+ * {16, 32, 64} (channels[0] = 16); input channels 1 (residual) or 2 (residual +
+ * nodal log-coefficient, or two displacement residual components); output
+ * channels = DOFs per node.  Activations bf16, bf16 MFMA with FP32
+ * accumulation; FP32 master parameters and Adam moments.                    */
+typedef struct apx_unet apx_unet;
+int apx_unet_create(apx_unet** out, int levels, const int* channels, int c_in, int c_out, int relu, int batch,
+                    int height, int width);
+int apx_unet_destroy(apx_unet* net);
+/* Parameter vector (FP32, host): per layer [co_pad x k_pad weights][co biases];
+ * apx_unet_layer describes layer i (weight W[co][t*ci + c], t = 3(dy+1)+(dx+1)). */
+int apx_unet_num_params(const apx_unet* net, int64_t* n_params, int* n_layers);
+int apx_unet_layer(const apx_unet* net, int layer, int64_t* w_off, int64_t* b_off, int* ci, int* co, int* co_pad,
+                   int* k_pad, int* relu, int* level);
+int apx_unet_set_params(apx_unet* net, const float* params_host);
+int apx_unet_get_params(const apx_unet* net, float* params_host);
+int apx_unet_get_grads(const apx_unet* net, float* grads_host);
+/* Raw buffers (device): input x (batch x H x W x c_in bf16), output y (FP32),
+ * per-image input scale (FP32).                                             */
+int apx_unet_buffers(apx_unet* net, void** x, void** y, void** scale);
+int apx_unet_forward(apx_unet* net, void* stream);
+/* loss = mean_b ||y_b - t_b||^2 / ||t_b||^2 for target t (device, FP32, layout of y);
+ * backward into the parameter gradients.                                    */
+int apx_unet_loss_backward(apx_unet* net, const float* target_dev, double* loss, void* stream);
+int apx_unet_adam(apx_unet* net, float lr, float beta1, float beta2, float eps, int step, void* stream);
+/* Network input of image `image` from a reduced FP64 residual r (n_free):
+ * x = r / s at free DOFs (0 at Dirichlet DOFs), s = RMS(r), plus the nodal
+ * field g_nodal (device, optional) as the last channel.  With e given, also the
+ * training target t = e / s (device FP32, layout of y).                    */
+int apx_unet_load_pair(apx_unet* net, apx_plan* plan, const double* r_dev, const double* g_nodal_dev,
+                       const double* e_dev, float* target_dev, int image, void* stream);
+/* One training step's loss and gradient, z_b = s_b net(r_b / s_b):
+ *   loss_kind 0 (residual): L = mean_b ||r_b - A z_b||^2 / ||r_b||^2
+ *   loss_kind 1 (energy):   L = mean_b sA (z_b.A z_b / 2 - z_b.r_b) / (s_b^2 n),
+ *                           sA = sign of A, minimizer z_b = A^-1 r_b
+ *   loss_kind 2 (A-norm):   L = mean_b (z_b-e_b).sA A (z_b-e_b) / e_b.sA A e_b,
+ *                           e_b = A^-1 r_b from e_list (required)
+ * plus w_e ||z_b - e_b||^2 / ||e_b||^2 when errors are given.  Loads the batch
+ * inputs from r_list[b] (device FP64 residuals, host array of `batch`
+ * pointers; g_list / e_list optional), runs forward and backward.  vals_dev:
+ * FP64 tangent in CSR order.  Follow with apx_unet_adam.                   */
+int apx_unet_train_step(apx_unet* net, apx_plan* plan, const void* vals_dev, const double* const* r_list,
+                        const double* const* g_list, const double* const* e_list, int loss_kind, double w_e,
+                        double* loss, void* stream);
+/* z = s * net(r / s) on the plan's free DOFs (FP64 in and out).            */
+int apx_unet_apply(apx_unet* net, apx_plan* plan, const double* r_dev, const double* g_nodal_dev, double* z_dev,
+                   void* stream);
+/* Use `net` as preconditioner of the plan (apx_newton with precond
+ * APX_PRECOND_LEARNED): flexible CG, Jacobi-PCG fallback on stagnation.     */
+int apx_plan_attach_unet(apx_plan* plan, apx_unet* net, const double* g_nodal_dev);
+/* One flexible-CG solve with the attached network (x0 = 0; vals in CSR order). */
+int apx_fcg(apx_plan* plan, const void* vals_dev, const double* b_dev, double* x_dev, int precision, double tol,
+            double atol, int maxiter, apx_krylov_stats* stats, void* stream);
+
 /* ---- misc ------------------------------------------------------------------ */
 const char* apx_last_error(void);
 int apx_version(void);

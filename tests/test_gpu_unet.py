@@ -1,0 +1,139 @@
+"""GPU tests of the learned preconditioner.
+
+- forward and parameter gradients of the HIP U-Net against the numpy oracle
+  (bf16 activations emulated in the oracle; tolerance: relative L2 error 2e-2
+  for outputs, 5e-2 per layer for gradients -- accumulation order and bf16
+  rounding-boundary effects only);
+- a network trained on the GPU on pairs from the plain solver needs fewer
+  Krylov iterations than Jacobi-PCG and reaches the SuperLU solution to 1e-8;
+- a broken network triggers the Jacobi-PCG fallback and Newton still converges.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import unet as OU
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _hip():
+    return C.CDLL("libamdhip64.so")
+
+
+def _copy(dst_ptr, src_ptr, nbytes):
+    rc = _hip().hipMemcpy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), C.c_size_t(nbytes), 3)  # device to device
+    assert rc == 0
+
+
+def _bf16_bits(a):
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+CASES = [(0, 1, 1), (1, 1, 1), (0, 2, 1), (1, 2, 2)]
+
+
+@pytest.mark.parametrize("relu,c_in,c_out", CASES)
+def test_forward_and_gradients_match_oracle(torch_gpu, relu, c_in, c_out):
+    torch = torch_gpu
+    from autopdex_amd.learned import UNetPreconditioner
+    H, W, L, ch, B = 37, 70, 3, (16, 32, 64), 2
+    print(f"[case] unet relu={relu} c_in={c_in} c_out={c_out}", flush=True)
+    net = UNetPreconditioner(H, W, L, ch, c_in, c_out, relu=bool(relu), batch=B, seed=5)
+    rng = np.random.default_rng(11)
+    theta = net.get_params()
+    for ly in net.layers:  # nonzero biases exercise the epilogue
+        theta[ly["b_off"]:ly["b_off"] + ly["co"]] = 0.05 * rng.standard_normal(ly["co"])
+    net.set_params(theta)
+    x = rng.standard_normal((B, H, W, c_in)).astype(np.float32)
+    xb = torch.from_numpy(_bf16_bits(x).view(np.int16)).cuda()
+    xp, yp, sp = net.buffers()
+    _copy(xp, xb.data_ptr(), xb.numel() * 2)
+    net.forward()
+    y = torch.empty(B * H * W * c_out, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    _copy(y.data_ptr(), yp, y.numel() * 4)
+    y = y.cpu().numpy().reshape(B, H, W, c_out)
+    orc = OU.UNetOracle(L, ch, c_in, c_out, relu, H, W, bf16_storage=True)
+    assert orc.n_params == net.n_params
+    y_ref = orc.forward(theta.astype(np.float64), OU.bf16(x))
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 2e-2
+    # gradients
+    t = rng.standard_normal((B, H, W, c_out)).astype(np.float32)
+    td = torch.from_numpy(t.ravel()).cuda()
+    loss = net.loss_backward(td)
+    g = net.get_grads()
+    loss_ref, g_ref = orc.loss_backward(t.astype(np.float64))
+    assert abs(loss - loss_ref) / loss_ref < 2e-2
+    for name, (w_off, b_off, ci, co, cop, wk) in orc.layout.items():
+        gw, gw_ref = g[w_off:b_off + co], g_ref[w_off:b_off + co]
+        err = np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30)
+        assert err < 5e-2, (name, err)
+
+
+def _poisson_problem(n):
+    from autopdex_amd import backend
+    dofs, settings, static, o = P.poisson_source(n, dict_dofs=False)
+    prob = backend.get_problem(dofs, settings, static)
+    return dofs, settings, static, o, prob
+
+
+def test_trained_network_reduces_iterations_and_matches_superlu(torch_gpu):
+    torch = torch_gpu
+    from autopdex_amd import learned
+    from oracle import assembly as OA
+    n = 64
+    dofs, settings, static, o, prob = _poisson_problem(n)
+    vals = learned.tangent(prob)
+    rng = np.random.default_rng(0)
+    pairs = []
+    for s in range(6):
+        g = P.grf(n + 1, 100 + s, length=0.1)[~o["mask"]] + 0.3 * rng.standard_normal(prob.n_free)
+        b = torch.from_numpy(g).cuda()
+        pairs += learned.solver_pairs(prob, vals, b, checkpoints=(0, 2, 8, 24, 64))
+    net = learned.UNetPreconditioner(n + 1, n + 1, 3, (16, 32, 64), 1, 1, relu=False, batch=4, seed=1)
+    hist = learned.train(net, prob, pairs, vals, steps=400, lr=3e-3, lr_final=3e-4, log=None, loss="residual")
+    assert hist[-1] < 0.5 * hist[0]
+    inf = learned.UNetPreconditioner(n + 1, n + 1, 3, (16, 32, 64), 1, 1, relu=False, batch=1)
+    inf.set_params(net.get_params())
+    inf.attach(prob)
+    # the benchmark right-hand side (short_example source) is not in the training set
+    r = torch.empty(prob.n_dofs, dtype=torch.float64, device="cuda")
+    from autopdex_amd import _native as N
+    from autopdex_amd.backend import current_stream
+    u0 = torch.zeros_like(r)
+    N.call("apx_assemble_residual", prob.plan, C.c_void_p(u0.data_ptr()), C.c_void_p(r.data_ptr()), current_stream())
+    r2f = torch.from_numpy(prob.free_dofs().astype(np.int64)).cuda()
+    b = (-r[r2f]).contiguous()
+    x_fcg, ks_f = learned.fcg(prob, vals, b, tol=1e-12, maxiter=2000)
+    x_pcg, ks_p = learned.pcg(prob, vals, b, tol=1e-12)
+    print(f"[gain] FCG+net {ks_f.iterations} it vs Jacobi-PCG {ks_p.iterations} it", flush=True)
+    assert ks_f.converged and ks_f.iterations < 0.5 * ks_p.iterations
+    A = OA.reduced_tangent(o["sets"], o["coords"], np.zeros(o["mask"].size), 1, ~o["mask"])
+    import scipy.sparse.linalg as spla
+    x_ref = spla.spsolve(A.tocsc(), b.cpu().numpy())
+    assert np.linalg.norm(x_fcg.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) < 1e-8
+
+
+def test_learned_newton_through_solver_and_fallback(torch_gpu):
+    torch = torch_gpu
+    from autopdex_amd import learned, solver
+    n = 32
+    dofs, settings, static, o, prob = _poisson_problem(n)
+    net = learned.UNetPreconditioner(n + 1, n + 1, 3, (16, 32, 64), 1, 1, relu=False, batch=1, seed=2)
+    net.attach(prob)
+    s = dict(static, **{"type of preconditioner": "learned", "linear tolerance": 1e-12,
+                        "newton tolerance mode": "relative"})
+    ref, _ = P.oracle_newton(o)
+    sol, infos = solver.solver(dofs, settings, s, newton_tol=1e-10)
+    assert infos[2] is False
+    assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
+    # a network that outputs zeros breaks flexible CG at once -> Jacobi-PCG fallback
+    net.set_params(np.zeros(net.n_params, dtype=np.float32))
+    sol, infos = solver.solver(dofs, settings, s, newton_tol=1e-10)
+    assert infos[2] is False and infos.details["fallbacks"] == 1
+    assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
