@@ -6,17 +6,21 @@
 // linear_solve_jax (solver.py:1044-1116) and the host spsolve of
 // linear_solve_scipy (:1493-1539) for the 'hip' backend.
 //
-// One PCG iteration = two kernels:
+// One PCG iteration = two streaming kernels, each followed by a one-block
+// finalize kernel:
 //   k_pcg_spmv  : p = z + beta p_old fused into the gather, q = A p, block
-//                 partials of p.q; the last block forms alpha = gamma / p.q.
+//                 partials of p.q;  k_pcg_alpha_fin forms alpha = gamma / p.q.
 //   k_pcg_update: x += alpha p, r -= alpha q, z = D^-1 r, partials of r.z and
-//                 r.r; the last block forms beta, gamma, the stop flag and the
-//                 reliable-update request.
+//                 r.r;  k_pcg_beta_fin forms beta, gamma, the stop flag and
+//                 the reliable-update request.
 // Scalars live on the device.  Kernels exit at entry once a flag is set, so
 // the host enqueues iterations in batches and reads the flags one batch late.
 // The two direction buffers alternate by the device iteration counter.
-// Reductions: fixed grid, per-block shuffle tree, the last block sums the
-// partials in index order -> bitwise reproducible.
+// Reductions: fixed grid, per-block shuffle tree, the finalize kernel sums
+// the partials in index order -> bitwise reproducible.  (A last-block-done
+// reduction inside the streaming kernel would need a device-scope fence per
+// block, which writes back the L2 of every XCD and cost more than the extra
+// launch.)
 //
 // Bandwidth: every thread issues kUnroll independent element loads per pass
 // (vector kernels) and one wavefront owns one 64-row SELL chunk with its

@@ -60,13 +60,24 @@ class UNetPreconditioner:
         self.set_params(self.initial_params(seed))
         self.step = 0
 
-    def initial_params(self, seed):
+    def up_layers(self):
+        """Indices of the coarse-to-fine ('up') convolutions; layer order: in, (e_l, down_l) for
+        l = 0..L-2, b, (up_l, d_l) for l = L-2..0, out."""
+        L = self.levels
+        return [2 * L + 2 * k for k in range(L - 1)]
+
+    def initial_params(self, seed, up_scale=1.0):
         """He-normal weights: std sqrt(gain / (9 ci)), gain 2 with ReLU else 1; zero biases.  The
-        output layer is scaled down so the untrained network starts close to zero."""
+        output layer is scaled down so the untrained network starts close to zero; up_scale < 1
+        starts the coarse branches small (full weighting sums 4 smooth values per level, so an
+        untrained deep branch amplifies smooth inputs by about 4 per level)."""
         rng = np.random.default_rng(seed)
         theta = np.zeros(self.n_params, dtype=np.float32)
+        ups = set(self.up_layers())
         for i, ly in enumerate(self.layers):
             std = np.sqrt((2.0 if ly["relu"] else 1.0) / (9 * ly["ci"]))
+            if i in ups:
+                std *= up_scale
             if i == self.n_layers - 1:
                 std *= 0.1
             W = np.zeros((ly["co_pad"], ly["k_pad"]), dtype=np.float32)
@@ -182,6 +193,82 @@ def tangent(problem, u=None):
     vals = torch.empty(problem.nnz, dtype=torch.float64, device=problem.device)
     N.call("apx_assemble_tangent", problem.plan, _vp(u), _vp(vals), N.APX_FP64, current_stream())
     return vals
+
+
+def random_field(height, width, length=0.05, sigma=1.0, seed=0, device="cuda"):
+    """Squared-exponential Gaussian random field on a (height x width) node grid of the
+    unit square, by circulant embedding (FFT on the doubled periodic grid), generated on
+    the device; returns a flat FP64 tensor in node order i * width + j."""
+    import torch
+    h = 1.0 / (max(height, width) - 1)
+    M0, M1 = 2 * height, 2 * width
+    k0 = torch.minimum(torch.arange(M0, device=device), M0 - torch.arange(M0, device=device)).double() * h
+    k1 = torch.minimum(torch.arange(M1, device=device), M1 - torch.arange(M1, device=device)).double() * h
+    cov = sigma ** 2 * torch.exp(-(k0[:, None] ** 2 + k1[None, :] ** 2) / (2 * length ** 2))
+    lam = torch.clamp(torch.fft.fft2(cov).real, min=0.0)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    xi = torch.complex(torch.randn(M0, M1, generator=gen, device=device, dtype=torch.float64),
+                       torch.randn(M0, M1, generator=gen, device=device, dtype=torch.float64))
+    f = torch.fft.fft2(torch.sqrt(lam / (M0 * M1)) * xi)
+    return f.real[:height, :width].contiguous().view(-1)
+
+
+def default_levels(height, width, coarsest=3):
+    """Levels of node-centred coarsening (H -> (H+1)/2) until the coarsest grid has about
+    `coarsest` nodes per side: the coarse levels must carry the smooth part of A^-1, so the
+    network reaches as deep as a multigrid hierarchy would (at most 12 levels)."""
+    L, h, w = 1, height, width
+    while L < 12 and min((h + 1) // 2, (w + 1) // 2) >= coarsest:
+        h, w, L = (h + 1) // 2, (w + 1) // 2, L + 1
+    return L
+
+
+def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=8, n_smooth=8,
+        checkpoints=(0, 16, 128), pair_tol=1e-9, steps=1500, batch=2, lr=1e-4, up_scale=0.05, seed=0,
+        log=None, time_budget=None):
+    """Train a linear U-Net preconditioner for `problem` (scalar field on a structured
+    height x width node grid) and attach it.
+
+    Training pairs are generated by the plain solver: Jacobi-PCG solves of A x = b for
+    white-noise and smooth random right-hand sides, sampled at the iterates in
+    `checkpoints` (r_k = b - A x_k, e_k = x* - x_k).  The loss is the relative A-norm error,
+    the quantity CG minimizes.  Returns (network with batch 1, attached; info dict)."""
+    torch = torch_cuda()
+    t0 = time.perf_counter()
+    if vals is None:
+        vals = tangent(problem)
+    L = levels or default_levels(height, width)
+    ch = tuple(channels or ((16, 32) + (64,) * 10)[:L])
+    free = torch.from_numpy(problem.free_dofs().astype(np.int64)).to(problem.device)
+    gen = torch.Generator(device=problem.device)
+    gen.manual_seed(int(seed))
+    rhs = [torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64)
+           for _ in range(n_noise)]
+    lengths = (0.2, 0.05, 0.02, 0.005)
+    rhs += [random_field(height, width, lengths[s % len(lengths)], seed=seed * 1000 + s)[free].contiguous()
+            for s in range(n_smooth)]
+    pairs = []
+    for b in rhs:
+        pairs += solver_pairs(problem, vals, b, checkpoints=checkpoints, tol=pair_tol)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    if log:
+        log(f"[fit] {len(pairs)} (r, e) pairs from {len(rhs)} plain solves in {t_pairs:.1f} s; levels {L}, "
+            f"channels {ch}")
+    net = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=batch, seed=seed + 1)
+    net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
+    t1 = time.perf_counter()
+    hist = train(net, problem, pairs, vals, steps=steps, lr=lr, lr_final=lr / 10, warmup=50, seed=seed,
+                 loss="anorm", log=log, log_every=max(1, steps // 5), time_budget=time_budget)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    inf = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=1)
+    inf.set_params(net.get_params())
+    inf.attach(problem)
+    info = dict(pairs=len(pairs), pair_seconds=t_pairs, train_seconds=t_train, steps=len(hist), levels=L,
+                channels=list(ch), final_loss=float(np.mean(hist[-50:])), n_params=int(inf.n_params))
+    return inf, info
 
 
 def solver_pairs(problem, vals, rhs, checkpoints=(0, 4, 16, 48, 128, 320, 800, 2000), tol=1e-10):

@@ -71,6 +71,12 @@ def options(static, method):
     elif method == "mixed":
         s["hip precision"] = "mixed"
         s["reliable delta"] = float(os.environ.get("APX_RELIABLE_DELTA", "0.1"))
+    elif method in ("learned", "learned_mixed"):
+        s["type of preconditioner"] = "learned"
+        s["linear tolerance"] = 1e-10
+        s["linear maxiter"] = 3000
+        if method == "learned_mixed":
+            s["hip precision"] = "mixed"
     else:
         raise ValueError(method)
     return solver.newton_options(s, newton_tol=1e-10, maxiter=100)
@@ -176,8 +182,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--methods", default="plain,mixed")
-    ap.add_argument("--headline", default="plain")
+    ap.add_argument("--methods", default="plain,mixed,learned_mixed")
+    ap.add_argument("--headline", default="learned_mixed")
+    ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="bounded kernel probe for counter runs")
     args = ap.parse_args()
@@ -198,6 +205,12 @@ def main():
     dv = torch.zeros(prob.n_dofs, dtype=torch.float64, device=dev)
     opts = {m: options(static, m) for m in methods}
     print(f"[bench] plan built in {plan_s:.2f} s: n_free={prob.n_free} nnz={prob.nnz}", file=sys.stderr, flush=True)
+    learned_info = None
+    if any(m.startswith("learned") for m in methods):
+        from autopdex_amd import learned
+        net, learned_info = learned.fit(prob, args.n + 1, args.n + 1, steps=args.train_steps,
+                                        log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(f"[bench] learned preconditioner: {learned_info}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         for m in methods:
             run(prob, opts[m], u, dv, torch)
@@ -238,6 +251,7 @@ def main():
                         "relative_residual": stats[m].residual_norm / stats[m].initial_residual,
                         "rel_error_vs_plain": errs.get(m, 0.0)} for m in methods},
         "plan_build_s": plan_s,
+        "learned_setup": learned_info,
         "assembly": {"residual_GBps": res.get("GBps"), "residual_ms": res.get("avg_ms"),
                      "residual_frac_hbm": res.get("frac_hbm"),
                      "dofs_assembled_per_s": prob.n_dofs / (res["avg_ms"] * 1e-3) if res else None,
