@@ -139,6 +139,11 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
     }
     double eta = o.lin_tol;
     if (o.forcing && res > 0.0) eta = std::max(eta, std::min(0.5, 0.5 * thr / res));
+    // an FP32 matrix bounds the true progress of one inner solve (about eps32 x cond(A)):
+    // stop each FP32 flexible-CG solve after a 'reliable delta' reduction and let the FP64
+    // Newton residual re-anchor the next one (flexible CG(1) keeps one direction, so the
+    // restart costs little)
+    if (learned && prec == APX_FP32 && o.reliable_delta > 0.0) eta = std::max(eta, o.reliable_delta);
     auto t0 = std::chrono::steady_clock::now();
     if (learned) {
       gather_rhs(P, C.r, b64.p, APX_FP64, C.st);

@@ -30,11 +30,19 @@ constexpr int kWgradBlocks = 512;
 int kblocks(int ci) { return ci >= 16 ? 9 * (ci / 16) : (9 * ci + 15) / 16; }
 
 // ---- conv dispatch ------------------------------------------------------------------
+// Grids with fewer tiles than kSplitTiles split the output channels over
+// 16-channel blocks (more blocks, 1/NT of the weights and MFMAs each).
+constexpr int64_t kSplitTiles = 128;
+
 template <int CI, int CO, bool F32>
 void launch_conv_t(const ConvArgs& a, hipStream_t st) {
+  constexpr int NT = round16(CO) / 16;
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, 2048));
-  hipLaunchKernelGGL((k_conv3x3<CI, CO, F32>), dim3(g), dim3(kConvThreads), 0, st, a);
+  if (NT > 1 && tiles < kSplitTiles)
+    hipLaunchKernelGGL((k_conv3x3<CI, CO, F32, 1>), dim3(g, NT), dim3(kConvThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_conv3x3<CI, CO, F32, NT>), dim3(g), dim3(kConvThreads), 0, st, a);
 }
 
 template <int CI, int CO>
@@ -83,53 +91,82 @@ void launch_wgrad(int ci, int co, const WgradArgs& a, hipStream_t st) {
 
 // ---- grid transfer ---------------------------------------------------------------------
 // P: fine(i, j) = sum_{I,J} w(i - 2I) w(j - 2J) coarse(I, J), w(0) = 1, w(+-1) = 1/2.
-// out = skip + P(coarse) (skip may alias out).
-__global__ void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip, uint16_t* out, int B, int H,
-                              int W, int Hc, int Wc, int C) {
-  const int64_t n = (int64_t)B * H * W * C;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    const int64_t p = e / C;
-    const int j = (int)(p % W), i = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
-    const int I0 = i >> 1, J0 = j >> 1;
-    const int I1 = (i & 1) ? I0 + 1 : I0, J1 = (j & 1) ? J0 + 1 : J0;
-    const uint16_t* cb = coarse + (int64_t)b * Hc * Wc * C;
-    auto at = [&](int I, int J) { return (I < Hc && J < Wc) ? bf2f(cb[((int64_t)I * Wc + J) * C + c]) : 0.f; };
-    float v;
-    if (!(i & 1) && !(j & 1))
-      v = at(I0, J0);
-    else if (!(i & 1))
-      v = 0.5f * (at(I0, J0) + at(I0, J1));
-    else if (!(j & 1))
-      v = 0.5f * (at(I0, J0) + at(I1, J0));
-    else
-      v = 0.25f * (at(I0, J0) + at(I0, J1) + at(I1, J0) + at(I1, J1));
-    if (skip) v += bf2f(skip[e]);
-    out[e] = f2bf(v);
+// out = skip + P(coarse) (skip may alias out).  One thread per (pixel, 8 channels):
+// 16-byte loads and stores, grid (row chunks, rows, images), 32-bit index math.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bf8_unpack(const u32x4& q, float f[8]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    f[2 * e] = __uint_as_float(q[e] << 16);
+    f[2 * e + 1] = __uint_as_float(q[e] & 0xffff0000u);
   }
+}
+__device__ __forceinline__ u32x4 bf8_pack(const float f[8]) {
+  u32x4 q;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) q[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+  return q;
+}
+
+template <int C8>
+__global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
+                                                     uint16_t* out, int H, int W, int Hc, int Wc) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= W * C8) return;
+  const int j = idx / C8, c = (idx % C8) * 8, i = blockIdx.y, b = blockIdx.z;
+  constexpr int C = 8 * C8;
+  const int I0 = i >> 1, J0 = j >> 1;
+  const int I1 = min((i & 1) ? I0 + 1 : I0, Hc - 1), J1 = min((j & 1) ? J0 + 1 : J0, Wc - 1);
+  const uint16_t* cb = coarse + (size_t)b * Hc * Wc * C + c;
+  float v[8], t[8];
+  auto acc = [&](int I, int J, float w) {
+    bf8_unpack(*(const u32x4*)(cb + ((size_t)I * Wc + J) * C), t);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] += w * t[e];
+  };
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = 0.f;
+  const bool oi = i & 1, oj = j & 1;
+  const bool hi = oi && I0 + 1 < Hc, hj = oj && J0 + 1 < Wc;  // outside the coarse grid: zero
+  const float wi0 = oi ? 0.5f : 1.f, wj0 = oj ? 0.5f : 1.f;
+  acc(I0, J0, wi0 * wj0);
+  if (hj) acc(I0, J1, wi0 * 0.5f);
+  if (hi) acc(I1, J0, 0.5f * wj0);
+  if (hi && hj) acc(I1, J1, 0.25f);
+  const size_t o = (((size_t)b * H + i) * W + j) * C + c;
+  if (skip) {
+    bf8_unpack(*(const u32x4*)(skip + o), t);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] += t[e];
+  }
+  *(u32x4*)(out + o) = bf8_pack(v);
 }
 
 // R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj).
-__global__ void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int B, int H, int W, int Hc,
-                           int Wc, int C) {
-  const int64_t n = (int64_t)B * Hc * Wc * C;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    const int64_t p = e / C;
-    const int J = (int)(p % Wc), I = (int)((p / Wc) % Hc), b = (int)(p / ((int64_t)Wc * Hc));
-    const uint16_t* fb = fine + (int64_t)b * H * W * C;
-    float v = 0.f;
+template <int C8>
+__global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int H,
+                                                  int W, int Hc, int Wc) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= Wc * C8) return;
+  const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
+  constexpr int C = 8 * C8;
+  const uint16_t* fb = fine + (size_t)b * H * W * C + c;
+  float v[8], t[8];
 #pragma unroll
-    for (int di = -1; di <= 1; ++di)
+  for (int e = 0; e < 8; ++e) v[e] = 0.f;
 #pragma unroll
-      for (int dj = -1; dj <= 1; ++dj) {
-        const int i = 2 * I + di, j = 2 * J + dj;
-        if (i < 0 || i >= H || j < 0 || j >= W) continue;
-        const float w = (di ? 0.5f : 1.f) * (dj ? 0.5f : 1.f);
-        v += w * bf2f(fb[((int64_t)i * W + j) * C + c]);
-      }
-    out[e] = f2bf(v);
-  }
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int i = 2 * I + di, j = 2 * J + dj;
+      if (i < 0 || i >= H || j < 0 || j >= W) continue;
+      const float w = (di ? 0.5f : 1.f) * (dj ? 0.5f : 1.f);
+      bf8_unpack(*(const u32x4*)(fb + ((size_t)i * W + j) * C), t);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] += w * t[e];
+    }
+  *(u32x4*)(out + (((size_t)b * Hc + I) * Wc + J) * C + c) = bf8_pack(v);
 }
 
 // ---- parameters ---------------------------------------------------------------------------
@@ -393,17 +430,33 @@ void conv_wgrad(UNet& N, int li, int level, const uint16_t* x, const uint16_t* d
 
 void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const uint16_t* skip, uint16_t* out,
                  hipStream_t st) {
-  int g;
-  grid1d(N.pixels(level_fine) * C, g);
-  hipLaunchKernelGGL(k_prolong_add, dim3(g), dim3(256), 0, st, coarse, skip, out, N.batch, N.H[level_fine],
-                     N.W[level_fine], N.H[level_fine + 1], N.W[level_fine + 1], C);
+  const int H = N.H[level_fine], W = N.W[level_fine];
+  APX_REQUIRE(C % 8 == 0 && H < 65536 && N.batch < 65536, APX_ERR_UNSUPPORTED, "grid transfer shape");
+  const dim3 grid((W * (C / 8) + 255) / 256, H, N.batch);
+  const int Hc = N.H[level_fine + 1], Wc = N.W[level_fine + 1];
+  if (C == 16)
+    hipLaunchKernelGGL(k_prolong_add<2>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc);
+  else if (C == 32)
+    hipLaunchKernelGGL(k_prolong_add<4>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc);
+  else if (C == 64)
+    hipLaunchKernelGGL(k_prolong_add<8>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc);
+  else
+    throw Error(APX_ERR_UNSUPPORTED, "grid transfer: channels must be 16, 32 or 64");
 }
 
 void restrict_(UNet& N, int level_fine, int C, const uint16_t* fine, uint16_t* out, hipStream_t st) {
-  int g;
-  grid1d(N.pixels(level_fine + 1) * C, g);
-  hipLaunchKernelGGL(k_restrict, dim3(g), dim3(256), 0, st, fine, out, N.batch, N.H[level_fine], N.W[level_fine],
-                     N.H[level_fine + 1], N.W[level_fine + 1], C);
+  const int H = N.H[level_fine], W = N.W[level_fine];
+  const int Hc = N.H[level_fine + 1], Wc = N.W[level_fine + 1];
+  APX_REQUIRE(C % 8 == 0 && Hc < 65536 && N.batch < 65536, APX_ERR_UNSUPPORTED, "grid transfer shape");
+  const dim3 grid((Wc * (C / 8) + 255) / 256, Hc, N.batch);
+  if (C == 16)
+    hipLaunchKernelGGL(k_restrict<2>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc);
+  else if (C == 32)
+    hipLaunchKernelGGL(k_restrict<4>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc);
+  else if (C == 64)
+    hipLaunchKernelGGL(k_restrict<8>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc);
+  else
+    throw Error(APX_ERR_UNSUPPORTED, "grid transfer: channels must be 16, 32 or 64");
 }
 }  // namespace
 

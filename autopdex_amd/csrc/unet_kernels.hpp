@@ -55,86 +55,132 @@ struct ConvArgs {
 // ---- forward / backward-data convolution ------------------------------------
 // CI, CO: stored channel counts.  K packing: CI >= 16 -> per tap CI/16 blocks of
 // 16 channels; CI < 16 -> k = t*CI + c over ceil(9 CI / 16) blocks.
-template <int CI, int CO, bool OUT_F32>
+//
+// Memory path: the input tile (one-pixel halo) and the weights are staged in
+// LDS with 16-byte global loads and 16-byte LDS stores (pixel stride CI + 8
+// elements keeps rows 16-byte aligned and the 8-byte fragment reads of the 16
+// lanes of a row group on distinct banks).  The accumulators go back through
+// LDS so that every wavefront stores its output row -- 64 pixels x CO channels,
+// contiguous in NHWC -- with 16-byte stores.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int CI>
+__host__ __device__ constexpr int conv_lds_stride() { return CI >= 16 ? CI + 8 : 8; }
+
+// NB: 16-wide output-channel tiles per block; blockIdx.y selects the channel
+// group (gridDim.y = NT / NB).  Small grids use NB = 1 so that more blocks
+// share the work and each stages only its own weight rows.  A block walks its
+// tiles with the next tile's input loads in flight (registers) during the
+// current tile's matrix work.
+template <int CI, int CO, bool OUT_F32, int NB>
 __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
-  constexpr int CIP = CI >= 16 ? CI : 16;
   constexpr int COP = round16(CO);
-  constexpr int S = lds_stride(CI);  // LDS pixel stride
+  constexpr int S = conv_lds_stride<CI>();
   constexpr int KB = CI >= 16 ? 9 * (CI / 16) : (9 * CI + 15) / 16;  // K blocks of 16
   constexpr int WK = KB * 16;                                        // packed K length (per co)
+  constexpr int SWK = WK + 8;                                        // LDS weight row stride
   constexpr int NT = COP / 16;
-  __shared__ uint16_t sx[(kTileH + 2) * (kTileW + 2) * S];
-  __shared__ uint16_t sw[COP * (WK + 4)];
-  // weights -> LDS once per block (layout [co][k], k packed as in the kernel's K order)
-  for (int i = threadIdx.x; i < COP * WK; i += kConvThreads) {
-    const int co = i / WK, k = i % WK;
-    sw[co * (WK + 4) + k] = a.w[i];
+  static_assert(NT % NB == 0, "channel groups must tile the output channels");
+  constexpr int CB = (NB == NT) ? CO : 16 * NB;  // output channels of this block
+  constexpr int SX = (kTileH + 2) * (kTileW + 2) * S;
+  constexpr int SO = kTileH * kTileW * CB * (OUT_F32 ? 2 : 1);  // output staging (uint16 units)
+  constexpr int SBUF = SX > SO ? SX : SO;
+  constexpr int CH8 = (CI + 7) / 8;  // 16-byte input chunks per pixel
+  constexpr int NP = (kTileH + 2) * (kTileW + 2);
+  constexpr int NC = NP * CH8, RC = (NC + kConvThreads - 1) / kConvThreads;
+  static_assert(OUT_F32 || CB % 8 == 0, "bf16 outputs are stored in 8-channel chunks");
+  __shared__ __attribute__((aligned(16))) uint16_t sbuf[SBUF];
+  __shared__ __attribute__((aligned(16))) uint16_t sw[16 * NB * SWK];
+  uint16_t* sx = sbuf;
+  const int co0 = blockIdx.y * 16 * NB;  // first output channel of this block
+  // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; all
+  // of a thread's loads are issued before its stores so that they overlap
+  {
+    constexpr int NW = 16 * NB * (WK / 8), RW = (NW + kConvThreads - 1) / kConvThreads;
+    u32x4 q[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      if (i < NW) q[r] = *(const u32x4*)(a.w + (int64_t)(co0 + i / (WK / 8)) * WK + (i % (WK / 8)) * 8);
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      if (i < NW) *(u32x4*)(sw + (i / (WK / 8)) * SWK + (i % (WK / 8)) * 8) = q[r];
+    }
   }
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + kTileH - 1) / kTileH;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lg = lane >> 4;
+  // input tile loads with a one-pixel halo (zero outside the image) into registers
+  u32x4 qs[RC];
+  auto load_tile = [&](int64_t tile) {
+    const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
+    const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
+    const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;
+    const int64_t img_off = (int64_t)img * a.H * a.W * CI;
+    const uint16_t* xb = a.x + img_off;
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      const int pix = i / CH8, ch = (i % CH8) * 8;
+      const int yy = ty0 - 1 + pix / (kTileW + 2), xx = tx0 - 1 + pix % (kTileW + 2);
+      const bool in = i < NC && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+      u32x4 q = u32x4{0u, 0u, 0u, 0u};
+      if (in) {
+        const int64_t o = ((int64_t)yy * a.W + xx) * CI + ch;
+        if constexpr (CI % 8 == 0) {
+          q = *(const u32x4*)(xb + o);
+          if (a.xmask) {
+            const u32x4 m = *(const u32x4*)(a.xmask + img_off + o);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              uint32_t keep = 0u;
+              if (bf2f((uint16_t)(m[e] & 0xffffu)) > 0.f) keep |= 0xffffu;
+              if (bf2f((uint16_t)(m[e] >> 16)) > 0.f) keep |= 0xffff0000u;
+              q[e] &= keep;
+            }
+          }
+        } else {
+          uint16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (ch + e < CI) {
+              v[e] = xb[o + e];
+              if (a.xmask && !(bf2f(a.xmask[img_off + o + e]) > 0.f)) v[e] = 0;
+            }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) q[e] = (uint32_t)v[2 * e] | ((uint32_t)v[2 * e + 1] << 16);
+        }
+      }
+      qs[r] = q;
+    }
+  };
+  if ((int64_t)blockIdx.x < ntiles) load_tile(blockIdx.x);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
     const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;
-    __syncthreads();  // previous tile's LDS reads done (also orders the weight load)
-    // stage the input tile with a one-pixel halo; zero outside the image
-    {
-      constexpr int CH8 = (CI + 7) / 8;  // 16-byte chunks per pixel
-      constexpr int NP = (kTileH + 2) * (kTileW + 2);
-      const uint16_t* xb = a.x + (int64_t)img * a.H * a.W * CI;
-      for (int i = threadIdx.x; i < NP * CH8; i += kConvThreads) {
-        const int pix = i / CH8, ch = (i % CH8) * 8;
-        const int yy = ty0 - 1 + pix / (kTileW + 2), xx = tx0 - 1 + pix % (kTileW + 2);
-        uint16_t v[8];
-        const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    __syncthreads();  // previous tile's output copy done (also orders the weight load)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = 0;
-        if (in) {
-          const int64_t o = ((int64_t)yy * a.W + xx) * CI + ch;
-          if constexpr (CI % 8 == 0) {
-            const uint4 q = *(const uint4*)(xb + o);
-            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
-            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
-            if (a.xmask) {
-              const uint4 m = *(const uint4*)(a.xmask + (int64_t)img * a.H * a.W * CI + o);
-              const uint32_t mm[4] = {m.x, m.y, m.z, m.w};
-#pragma unroll
-              for (int e = 0; e < 8; ++e) {
-                const uint16_t me = (uint16_t)(mm[e / 2] >> (16 * (e & 1)));
-                if (!(bf2f(me) > 0.f)) v[e] = 0;
-              }
-            }
-          } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-              if (ch + e < CI) {
-                v[e] = xb[o + e];
-                if (a.xmask && !(bf2f(a.xmask[(int64_t)img * a.H * a.W * CI + o + e]) > 0.f)) v[e] = 0;
-              }
-          }
-        }
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-          if (ch + e < S) sx[pix * S + ch + e] = v[e];
-      }
+    for (int r = 0; r < RC; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      if (i < NC) *(u32x4*)(sx + (i / CH8) * S + (i % CH8) * 8) = qs[r];
     }
     __syncthreads();
-    // each wavefront: one tile row, 4 row tiles of 16 pixels, NT column tiles
-    f32x4 acc[4][NT];
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);  // in flight during the MFMAs
+    // each wavefront: one tile row, 4 row tiles of 16 pixels, NB column tiles
+    f32x4 acc[4][NB];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int n = 0; n < NB; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
-      bf16x4 bfrag[NT];
+      bf16x4 bfrag[NB];
 #pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        const uint16_t* wp = sw + (n * 16 + lr) * (WK + 4) + kb * 16 + 4 * lg;
-        bfrag[n] = bf16x4{(short)wp[0], (short)wp[1], (short)wp[2], (short)wp[3]};
-      }
+      for (int n = 0; n < NB; ++n) bfrag[n] = *(const bf16x4*)(sw + (n * 16 + lr) * SWK + kb * 16 + 4 * lg);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         bf16x4 af;
@@ -142,8 +188,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
         if constexpr (CI >= 16) {
           const int t = kb / (CI / 16), cb = (kb % (CI / 16)) * 16;
           const int dy = t / 3, dx = t % 3;  // halo offsets 0..2
-          const uint16_t* xp = sx + ((wave + dy) * (kTileW + 2) + px + dx) * S + cb + 4 * lg;
-          af = bf16x4{(short)xp[0], (short)xp[1], (short)xp[2], (short)xp[3]};
+          af = *(const bf16x4*)(sx + ((wave + dy) * (kTileW + 2) + px + dx) * S + cb + 4 * lg);
         } else {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
@@ -157,32 +202,49 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
           }
         }
 #pragma unroll
-        for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfrag[n], acc[m][n], 0, 0, 0);
+        for (int n = 0; n < NB; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfrag[n], acc[m][n], 0, 0, 0);
       }
     }
-    // epilogue: bias, activation, store (lane holds column co = n*16+lr, pixels 4*lg+j)
-    const int yy = ty0 + wave;
-    if (yy < a.H) {
+    __syncthreads();  // every wavefront is done with the input tile: reuse it for the output rows
+    // epilogue: bias, activation -> LDS row of this wavefront (lane holds co = co0+n*16+lr, pixels 4*lg+j)
+    {
       const float scale = (OUT_F32 && a.yscale) ? a.yscale[img] : 1.f;
 #pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        const int co = n * 16 + lr;
-        if (co >= CO) continue;
-        const float bb = a.bias ? a.bias[co] : 0.f;
+      for (int n = 0; n < NB; ++n) {
+        const int cl = n * 16 + lr;  // channel inside the block's group
+        if (cl < CB && co0 + cl < CO) {
+          const float bb = a.bias ? a.bias[co0 + cl] : 0.f;
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+          for (int m = 0; m < 4; ++m)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int xx = tx0 + m * 16 + 4 * lg + j;
-            if (xx >= a.W) continue;
-            float v = acc[m][n][j] + bb;
-            if (a.relu) v = fmaxf(v, 0.f);
-            const int64_t o = (((int64_t)img * a.H + yy) * a.W + xx) * CO + co;
-            if constexpr (OUT_F32)
-              ((float*)a.y)[o] = v * scale;
-            else
-              ((uint16_t*)a.y)[o] = f2bf(v);
-          }
+            for (int j = 0; j < 4; ++j) {
+              float v = acc[m][n][j] + bb;
+              if (a.relu) v = fmaxf(v, 0.f);
+              const int px = m * 16 + 4 * lg + j;
+              if constexpr (OUT_F32)
+                ((float*)sbuf)[(wave * kTileW + px) * CB + cl] = v * scale;
+              else
+                sbuf[(wave * kTileW + px) * CB + cl] = f2bf(v);
+            }
+        }
+      }
+    }
+    __syncthreads();
+    // copy out: each wavefront's row holds 64 pixels x CB channels (contiguous in NHWC when CB = CO)
+    const int yy = ty0 + wave;
+    const int npx = min(kTileW, a.W - tx0);
+    if (yy < a.H) {
+      const int64_t row0 = (((int64_t)img * a.H + yy) * a.W + tx0) * CO + co0;
+      if constexpr (OUT_F32) {
+        const float* src = (const float*)sbuf + wave * kTileW * CB;
+        float* dst = (float*)a.y + row0;
+        for (int i = lane; i < npx * CB; i += 64) dst[(i / CB) * CO + i % CB] = src[i];
+      } else {
+        constexpr int C8 = CB / 8;  // 16-byte chunks per pixel of this group
+        const uint16_t* src = sbuf + wave * kTileW * CB;
+        uint16_t* dst = (uint16_t*)a.y + row0;
+        for (int i = lane; i < npx * C8; i += 64)
+          *(u32x4*)(dst + (i / C8) * CO + (i % C8) * 8) = *(const u32x4*)(src + 8 * i);
       }
     }
   }

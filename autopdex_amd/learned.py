@@ -224,16 +224,17 @@ def default_levels(height, width, coarsest=3):
     return L
 
 
-def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=8, n_smooth=8,
-        checkpoints=(0, 16, 128), pair_tol=1e-9, steps=1500, batch=2, lr=1e-4, up_scale=0.05, seed=0,
+def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=16, n_smooth=16,
+        pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
         log=None, time_budget=None):
     """Train a linear U-Net preconditioner for `problem` (scalar field on a structured
     height x width node grid) and attach it.
 
-    Training pairs are generated by the plain solver: Jacobi-PCG solves of A x = b for
-    white-noise and smooth random right-hand sides, sampled at the iterates in
-    `checkpoints` (r_k = b - A x_k, e_k = x* - x_k).  The loss is the relative A-norm error,
-    the quantity CG minimizes.  Returns (network with batch 1, attached; info dict)."""
+    Training pairs come from the plain solver: Jacobi-PCG solves of A e = r for white-noise
+    and smooth random right-hand sides r (squared-exponential fields of lengths 0.2 ... 0.005).
+    The loss is the relative A-norm error, the quantity CG minimizes.  Every `validate_every`
+    steps the parameters are scored by flexible CG (to 1e-6) on a held-out right-hand side and
+    the best ones are kept.  Returns (network with batch 1, attached; info dict)."""
     torch = torch_cuda()
     t0 = time.perf_counter()
     if vals is None:
@@ -243,31 +244,55 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=8
     free = torch.from_numpy(problem.free_dofs().astype(np.int64)).to(problem.device)
     gen = torch.Generator(device=problem.device)
     gen.manual_seed(int(seed))
-    rhs = [torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64)
-           for _ in range(n_noise)]
     lengths = (0.2, 0.05, 0.02, 0.005)
-    rhs += [random_field(height, width, lengths[s % len(lengths)], seed=seed * 1000 + s)[free].contiguous()
-            for s in range(n_smooth)]
+
+    def noise():
+        return torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64)
+
+    def smooth(s):
+        return random_field(height, width, lengths[s % len(lengths)], seed=seed * 1000 + s)[free].contiguous()
+
+    rhs = [noise() for _ in range(n_noise)] + [smooth(s) for s in range(n_smooth)]
     pairs = []
     for b in rhs:
-        pairs += solver_pairs(problem, vals, b, checkpoints=checkpoints, tol=pair_tol)
+        pairs += solver_pairs(problem, vals, b, checkpoints=(0,), tol=pair_tol)
+    b_val = noise() + smooth(n_smooth + 1) * 10.0  # held out
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
     if log:
-        log(f"[fit] {len(pairs)} (r, e) pairs from {len(rhs)} plain solves in {t_pairs:.1f} s; levels {L}, "
+        log(f"[fit] {len(pairs)} (r, A^-1 r) pairs from {len(rhs)} plain solves in {t_pairs:.1f} s; levels {L}, "
             f"channels {ch}")
     net = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=batch, seed=seed + 1)
     net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
+    inf = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=1)
+    inf.attach(problem)
     t1 = time.perf_counter()
-    hist = train(net, problem, pairs, vals, steps=steps, lr=lr, lr_final=lr / 10, warmup=50, seed=seed,
-                 loss="anorm", log=log, log_every=max(1, steps // 5), time_budget=time_budget)
+    best, history, done = None, [], 0
+    seg = validate_every if validate_every else steps
+    while done < steps:
+        k = min(seg, steps - done)
+        lr0 = lr * (0.1 ** (done / steps))
+        lr1 = lr * (0.1 ** ((done + k) / steps))
+        history += train(net, problem, pairs, vals, steps=k, lr=lr0, lr_final=lr1,
+                         warmup=50 if done == 0 else 0, seed=seed + done, loss="anorm", log=None)
+        done += k
+        theta = net.get_params()
+        inf.set_params(theta)
+        _, ks = fcg(problem, vals, b_val, tol=1e-6, maxiter=400)
+        score = ks.iterations if ks.converged else 10 ** 6
+        if log:
+            log(f"[fit] step {done}: loss {np.mean(history[-50:]):.3e}, validation FCG {ks.iterations} it"
+                f"{'' if ks.converged else ' (not converged)'}")
+        if best is None or score <= best[0]:
+            best = (score, theta, done)
+        if time_budget and time.perf_counter() - t1 > time_budget:
+            break
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
-    inf = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=1)
-    inf.set_params(net.get_params())
-    inf.attach(problem)
-    info = dict(pairs=len(pairs), pair_seconds=t_pairs, train_seconds=t_train, steps=len(hist), levels=L,
-                channels=list(ch), final_loss=float(np.mean(hist[-50:])), n_params=int(inf.n_params))
+    inf.set_params(best[1])
+    info = dict(pairs=len(pairs), pair_seconds=t_pairs, train_seconds=t_train, steps=done, best_step=best[2],
+                validation_iterations=best[0], levels=L, channels=list(ch), batch=batch,
+                final_loss=float(np.mean(history[-50:])), n_params=int(inf.n_params))
     return inf, info
 
 

@@ -77,6 +77,7 @@ def options(static, method):
         s["linear maxiter"] = 3000
         if method == "learned_mixed":
             s["hip precision"] = "mixed"
+            s["reliable delta"] = float(os.environ.get("APX_LEARNED_DELTA", "1e-2"))
     else:
         raise ValueError(method)
     return solver.newton_options(s, newton_tol=1e-10, maxiter=100)
@@ -185,6 +186,7 @@ def main():
     ap.add_argument("--methods", default="plain,mixed,learned_mixed")
     ap.add_argument("--headline", default="learned_mixed")
     ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--seed", type=int, default=0, help="seed of the preconditioner's training data")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="bounded kernel probe for counter runs")
     args = ap.parse_args()
@@ -208,7 +210,7 @@ def main():
     learned_info = None
     if any(m.startswith("learned") for m in methods):
         from autopdex_amd import learned
-        net, learned_info = learned.fit(prob, args.n + 1, args.n + 1, steps=args.train_steps,
+        net, learned_info = learned.fit(prob, args.n + 1, args.n + 1, steps=args.train_steps, seed=args.seed,
                                         log=lambda m: print(m, file=sys.stderr, flush=True))
         print(f"[bench] learned preconditioner: {learned_info}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
