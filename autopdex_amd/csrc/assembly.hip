@@ -157,39 +157,100 @@ static void dispatch(Plan& P, ElemSet& S, const double* u, double* r, void* vals
   }
 }
 
-// Algorithmic (compulsory) bytes of one assembly: coordinates, DOFs and
-// connectivity read once, outputs written once.
+void assemble_set_colour_residual(Plan& P, ElemSet& S, const double* u, double* r, hipStream_t st) {
+  dispatch<double, MODE_RES>(P, S, u, r, nullptr, st);
+}
+
+// Algorithmic (compulsory) bytes of one assembly: what the kernels that run
+// must move at least once -- coordinates and DOFs read, outputs written; the
+// colour path also reads the connectivity (and, for the tangent, the slot
+// map is not counted: it is an implementation choice), the owner-tile path
+// derives the connectivity from the grid and reads the precomputed source
+// part instead.
 int64_t residual_bytes(const Plan& P) {
   int64_t b = P.n_nodes * P.dim * 8 + P.n_dofs * 8 * 2;
-  for (auto& s : P.sets) b += s.n_elems * s.npe * 4;
+  for (auto& s : P.sets) {
+    if (tile_supported(P, s, 0))
+      b += s.model == APX_MODEL_POISSON_SOURCE ? P.n_dofs * 8 : 0;
+    else
+      b += s.n_elems * s.npe * 4;
+    if (s.model == APX_MODEL_POISSON_VARK) b += P.n_nodes * 16;
+  }
   return b;
 }
 
 int64_t tangent_bytes(const Plan& P, int precision) {
-  int64_t b = P.n_nodes * P.dim * 8 + P.n_dofs * 8 + P.nnz * (precision == APX_FP64 ? 8 : 4);
-  for (auto& s : P.sets) b += s.n_elems * s.npe * 4;
+  const int vb = precision == APX_FP64 ? 8 : 4;
+  int64_t b = P.n_nodes * P.dim * 8 + P.n_dofs * 8;
+  if (tangent_by_tiles(P))
+    b += P.sell_total * vb + P.n_free * 4 + (P.n_chunks + 1) * 4 + P.n_dofs;  // values incl. padding, maps, mask
+  else
+    b += P.nnz * vb;
+  for (auto& s : P.sets) {
+    if (!tangent_by_tiles(P)) b += s.n_elems * s.npe * 4;
+    if (s.model == APX_MODEL_POISSON_VARK) b += P.n_nodes * 16;
+  }
   return b;
 }
 
+// One set carries the whole tangent, on a structured grid whose reduced pattern
+// is the 3 x 3 node stencil: the owner-tile kernel writes every stored entry.
+bool tangent_by_tiles(const Plan& P) {
+  if (!P.stencil_pattern) return false;
+  const ElemSet* only = nullptr;
+  for (auto& s : P.sets) {
+    if (s.model == APX_MODEL_LINE_TRACTION) continue;  // zero tangent
+    if (only) return false;
+    only = &s;
+  }
+  return only && tile_supported(P, *only, 1);
+}
+
 void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st) {
-  fill<double>(r, P.n_dofs, 0.0, st);
-  for (auto& S : P.sets) dispatch<double, MODE_RES>(P, S, u, r, nullptr, st);
+  bool have = false;  // r holds this assembly's partial sums
+  for (auto& S : P.sets) {
+    if (tile_supported(P, S, 0)) {
+      tile_residual(P, S, u, r, !have, st);
+    } else {
+      if (!have) fill<double>(r, P.n_dofs, 0.0, st);
+      dispatch<double, MODE_RES>(P, S, u, r, nullptr, st);
+    }
+    have = true;
+  }
+  if (!have) fill<double>(r, P.n_dofs, 0.0, st);
 }
 
 // Tangent values in the plan's SELL layout (P.sell_total entries, padding 0).
 void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st) {
+  if (tangent_by_tiles(P)) {
+    for (auto& S : P.sets)
+      if (S.model != APX_MODEL_LINE_TRACTION) tile_tangent(P, S, u, vals, precision, st);
+    return;
+  }
   if (precision == APX_FP64) {
     fill<double>((double*)vals, P.sell_total, 0.0, st);
-    for (auto& S : P.sets) dispatch<double, MODE_TAN>(P, S, u, nullptr, vals, st);
+    for (auto& S : P.sets)
+      if (S.model != APX_MODEL_LINE_TRACTION) dispatch<double, MODE_TAN>(P, S, u, nullptr, vals, st);
   } else {
     fill<float>((float*)vals, P.sell_total, 0.0f, st);
-    for (auto& S : P.sets) dispatch<float, MODE_TAN>(P, S, u, nullptr, vals, st);
+    for (auto& S : P.sets)
+      if (S.model != APX_MODEL_LINE_TRACTION) dispatch<float, MODE_TAN>(P, S, u, nullptr, vals, st);
   }
 }
 
 void assemble_diagonal(Plan& P, const double* u, double* diag, hipStream_t st) {
-  fill<double>(diag, P.n_dofs, 0.0, st);
-  for (auto& S : P.sets) dispatch<double, MODE_DIAG>(P, S, u, diag, nullptr, st);
+  bool have = false;
+  for (auto& S : P.sets) {
+    if (S.model == APX_MODEL_LINE_TRACTION) continue;  // zero tangent
+    if (tile_supported(P, S, 2)) {
+      tile_diagonal(P, S, u, diag, !have, st);
+    } else {
+      if (!have) fill<double>(diag, P.n_dofs, 0.0, st);
+      dispatch<double, MODE_DIAG>(P, S, u, diag, nullptr, st);
+    }
+    have = true;
+  }
+  if (!have) fill<double>(diag, P.n_dofs, 0.0, st);
 }
 
 }  // namespace apx

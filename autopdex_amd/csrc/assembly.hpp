@@ -12,6 +12,16 @@ void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipSt
 void assemble_diagonal(Plan& P, const double* u, double* diag, hipStream_t st);
 int64_t residual_bytes(const Plan& P);
 int64_t tangent_bytes(const Plan& P, int precision);
+// one set's residual through the colour kernels (r += ...)
+void assemble_set_colour_residual(Plan& P, ElemSet& S, const double* u, double* r, hipStream_t st);
+
+// structured.hip: owner-tile kernels for structured Q1 grids
+// mode: 0 residual, 1 tangent, 2 diagonal
+bool tile_supported(const Plan& P, const ElemSet& S, int mode);
+bool tangent_by_tiles(const Plan& P);
+void tile_residual(Plan& P, ElemSet& S, const double* u, double* r, bool overwrite, hipStream_t st);
+void tile_tangent(Plan& P, ElemSet& S, const double* u, void* vals, int precision, hipStream_t st);
+void tile_diagonal(Plan& P, ElemSet& S, const double* u, double* diag, bool overwrite, hipStream_t st);
 
 // krylov.hip
 struct KState {

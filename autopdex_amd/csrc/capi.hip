@@ -85,6 +85,7 @@ int apx_plan_set_params(apx_plan* plan, int set, const double* params, int n_par
     APX_REQUIRE(set >= 0 && set < (int)plan->sets.size(), APX_ERR_ARG, "set index out of range");
     APX_REQUIRE(n_params >= 0 && n_params <= kMaxParams, APX_ERR_ARG, "too many parameters");
     for (int i = 0; i < n_params; ++i) plan->sets[set].params.p[i] = params[i];
+    plan->sets[set].load_valid = false;  // the precomputed source part depends on the parameters
   });
 }
 
@@ -129,6 +130,9 @@ int apx_plan_get_info(const apx_plan* plan, apx_plan_info* info) {
     info->dofs_per_node = plan->d;
     info->n_sets = (int)plan->sets.size();
     info->structured = 0;
+    for (auto& s : plan->sets)
+      if (s.structured) info->structured |= 1;
+    if (plan->finalized && tangent_by_tiles(*plan)) info->structured |= 2;
     for (size_t i = 0; i < plan->sets.size() && i < 8; ++i)
       info->n_colours[i] = plan->sets[i].colour_offsets.empty() ? 0 : (int)plan->sets[i].colour_offsets.size() - 1;
     info->build_seconds = plan->build_seconds;

@@ -174,6 +174,30 @@ struct PoissonSource {
         out[0][b] += Q[q].wdet * gg;
       }
   }
+  // Gauss point q's contribution to every node (one thread per element)
+  __device__ static void res_gp(int q, const QP<double>& Q, const ModelParams&, double out[NPE][1]) {
+    double N[NPE], dN[NPE][DIM];
+    Shape<NPE, DIM>::eval(q, N, dN);
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+      double gg = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) gg += Q.G[a][k] * Q.grad[k];
+      out[a][0] += Q.wdet * (gg - Q.srcN * N[a]);
+    }
+  }
+  template <class T>
+  __device__ static void tan_gp(const QP<T>& Q, const ModelParams&, T K[NPE][NPE]) {
+#pragma unroll
+    for (int a = 0; a < NPE; ++a)
+#pragma unroll
+      for (int b = 0; b < NPE; ++b) {
+        T gg = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) gg += Q.G[a][k] * Q.G[b][k];
+        K[a][b] += Q.wdet * gg;
+      }
+  }
 };
 
 // ---- Poisson with nodal coefficient k = exp(N.g) and source f = N.f -------------
@@ -236,6 +260,30 @@ struct PoissonVarK {
         out[0][b] += w * gg;
       }
     }
+  }
+  __device__ static void res_gp(int q, const QP<double>& Q, const ModelParams&, double out[NPE][1]) {
+    double N[NPE], dN[NPE][DIM];
+    Shape<NPE, DIM>::eval(q, N, dN);
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+      double gg = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) gg += Q.G[a][k] * Q.grad[k];
+      out[a][0] += Q.wdet * (Q.kq * gg - Q.fq * N[a]);
+    }
+  }
+  template <class T>
+  __device__ static void tan_gp(const QP<T>& Q, const ModelParams&, T K[NPE][NPE]) {
+    const T w = Q.wdet * Q.kq;
+#pragma unroll
+    for (int a = 0; a < NPE; ++a)
+#pragma unroll
+      for (int b = 0; b < NPE; ++b) {
+        T gg = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) gg += Q.G[a][k] * Q.G[b][k];
+        K[a][b] += w * gg;
+      }
   }
 };
 
@@ -309,6 +357,12 @@ struct NeoHooke {
         o.H[a][k] = (T)h;
         o.FG[a][k] = (T)(want_res ? mu * fg : 0.0);
       }
+  }
+  __device__ static void res_gp(int, const QP<double>& Q, const ModelParams&, double out[NPE][DIM]) {
+#pragma unroll
+    for (int a = 0; a < NPE; ++a)
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) out[a][i] += Q.wdet * (Q.FG[a][i] - Q.c1 * Q.H[a][i]);
   }
   // r_ai = sum_q w det (mu (F G_a)_i - c1 H_ai)
   __device__ static void res_row(int a, const QP<double>* Q, const ModelParams&, double out[DIM]) {

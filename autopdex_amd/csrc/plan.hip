@@ -170,6 +170,52 @@ void plan_build_host(Plan& P, std::vector<std::vector<int32_t>>& conn_perm,
     }
   }
 
+  // 4b. structured Q1 grids (owner-tile kernels)
+  for (auto& s : P.sets) {
+    s.structured = 0;
+    if (P.dim != 2 || s.npe != 4 || s.n_elems < 1 || s.conn_host[0] != 0) continue;
+    const int64_t ny = s.conn_host[3] - 1;
+    if (ny < 1 || s.n_elems % ny) continue;
+    const int64_t nx = s.n_elems / ny;
+    if ((nx + 1) * (ny + 1) != P.n_nodes) continue;
+    bool ok = true;
+    for (int64_t e = 0; e < s.n_elems && ok; ++e) {
+      const int64_t i = e / ny, j = e % ny, b = i * (ny + 1) + j;
+      const int64_t* c = &s.conn_host[e * 4];
+      ok = c[0] == b && c[1] == b + 1 && c[2] == b + ny + 2 && c[3] == b + ny + 1;
+    }
+    if (ok) {
+      s.structured = 1;
+      s.gnx = nx;
+      s.gny = ny;
+    }
+  }
+  // 4c. is the reduced pattern the free-node 3 x 3 stencil of that grid (one DOF per node)?
+  P.stencil_pattern = false;
+  for (auto& s : P.sets) {
+    if (!s.structured || P.d != 1) continue;
+    const int64_t nx = s.gnx, ny = s.gny;
+    bool ok = true;
+    for (int64_t I = 0; I <= nx && ok; ++I)
+      for (int64_t J = 0; J <= ny && ok; ++J) {
+        const int32_t row = P.full2red_h[I * (ny + 1) + J];
+        if (row < 0) continue;
+        int32_t k = P.indptr_h[row];
+        for (int dI = -1; dI <= 1 && ok; ++dI)
+          for (int dJ = -1; dJ <= 1 && ok; ++dJ) {
+            const int64_t I2 = I + dI, J2 = J + dJ;
+            if (I2 < 0 || I2 > nx || J2 < 0 || J2 > ny) continue;
+            const int32_t col = P.full2red_h[I2 * (ny + 1) + J2];
+            if (col < 0) continue;
+            ok = k < P.indptr_h[row + 1] && P.indices_h[k] == col;
+            ++k;
+          }
+        ok = ok && k == P.indptr_h[row + 1];
+      }
+    P.stencil_pattern = ok;
+    break;
+  }
+
   // 5. greedy element colouring per set (node-disjoint colours) and slot maps
   //    (slots index the SELL layout)
   conn_perm.assign(P.sets.size(), {});

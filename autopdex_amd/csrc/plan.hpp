@@ -25,6 +25,13 @@ struct ElemSet {
   DevBuf<int32_t> conn;            // n_elems * npe, colour-major order
   DevBuf<int32_t> slots;           // n_elems * (npe*d)^2 reduced CSR slots, -1 = Dirichlet
   std::vector<int64_t> colour_offsets;  // n_colours + 1
+  // Structured Q1 grid (the connectivity of mesher.structured_mesh): element
+  // e = i * gny + j has nodes b, b + 1, b + gny + 2, b + gny + 1 with
+  // b = i * (gny + 1) + j.  Such sets use the owner-tile kernels.
+  int structured = 0;
+  int64_t gnx = 0, gny = 0;
+  DevBuf<double> load;     // u-independent residual part (precomputed-source models), n_dofs
+  bool load_valid = false;
 };
 
 // Timing of a kernel family: HIP events on the caller's stream.
@@ -59,6 +66,7 @@ struct Plan {
   // longest row).  Element slot maps point into this layout; the C ABI
   // exchanges values in canonical CSR order through csr2sell.
   int64_t sell_total = 0, n_chunks = 0;
+  bool stencil_pattern = false;  // reduced rows = free nodes of the 3 x 3 stencil of a structured grid
   std::vector<int32_t> sell_ptr_h, sell_cols_h, csr2sell_h;
   DevBuf<int32_t> sell_ptr;    // n_chunks + 1 (entry offsets)
   DevBuf<int32_t> sell_cols;   // sell_total (padding: column 0, value 0)

@@ -79,6 +79,10 @@ int apx_plan_finalize(apx_plan* plan);
 int apx_plan_build_pattern(apx_plan* plan);
 int apx_plan_destroy(apx_plan* plan);
 
+/* structured: bit 0 = a set is a structured Q1 grid (mesher.structured_mesh
+ * connectivity) and uses the owner-tile residual kernels; bit 1 = the tangent
+ * is assembled by the owner-tile kernels too (one DOF per node, stencil
+ * pattern).                                                                  */
 typedef struct apx_plan_info {
   int64_t n_nodes, n_dofs, n_free, nnz;
   int dim, dofs_per_node, n_sets, structured;

@@ -48,6 +48,22 @@ def poisson_source(n, dict_dofs=True):
     return dofs, settings, static([pot]), oracle
 
 
+def poisson_rect(nx, ny, verts=((0.0, 0.0), (2.0, 0.0), (2.0, 0.7), (0.0, 1.0))):
+    """Poisson with source on an nx x ny Q1 quadrilateral (non-square grid, skewed
+    elements), psdf-polygon Dirichlet mask with free corners."""
+    verts = [list(v) for v in verts]
+    coords, elems = mesher.structured_mesh((nx, ny), verts, "quad")
+    mask = geometry.in_sdfs(coords, lambda x: geometry.psdf_polygon(x, verts))
+    pot = models.mixed_reference_domain_potential(
+        models.poisson_source_integrand(), {"phi": spaces.fem_iso_line_quad_brick},
+        *seeder.gauss_legendre_nd(dimension=2, order=2), "phi")
+    settings = {"connectivity": (elems,), "dirichlet dofs": mask, "node coordinates": coords,
+                "dirichlet conditions": np.zeros(mask.shape)}
+    oracle = dict(coords=coords, sets=[(OE.PoissonSource(), elems, None)], mask=mask.ravel(),
+                  values=np.zeros(mask.size), d=1)
+    return np.zeros(mask.shape), settings, static([pot]), oracle
+
+
 def grf(n_nodes_1d, seed, length=0.05, sigma=1.0):
     """Squared-exponential Gaussian random field on an (m x m) node grid of [0,1]^2
     by circulant embedding (periodic FFT on a doubled grid), seeded PCG64."""

@@ -32,7 +32,17 @@ CASES = [
     ("poisson_vark_q1", P.poisson_vark, (12, 3)),
     ("neo_hooke_hex8", P.neo_hooke_cube, (4,)),
     ("neo_hooke_plane_strain_traction", P.tapered_panel, (8,)),
+    # several owner tiles in each direction (tiles own 4 x 63 nodes), partial edge tiles
+    ("poisson_q1_tiles", P.poisson_source, (130,)),
+    ("poisson_rect_tiles", P.poisson_rect, (9, 200)),
+    ("poisson_rect_tiles_t", P.poisson_rect, (70, 5)),
+    ("poisson_vark_q1_tiles", P.poisson_vark, (70, 5)),
+    ("neo_hooke_plane_strain_tiles", P.tapered_panel, (66,)),
 ]
+# bit 0: owner-tile residual kernels in use; bit 1: owner-tile tangent too
+STRUCTURED = {"poisson_q1": 3, "poisson_vark_q1": 3, "neo_hooke_hex8": 0, "neo_hooke_plane_strain_traction": 1,
+              "poisson_q1_tiles": 3, "poisson_rect_tiles": 3, "poisson_rect_tiles_t": 3, "poisson_vark_q1_tiles": 3,
+              "neo_hooke_plane_strain_tiles": 1}
 
 
 @pytest.mark.parametrize("name,builder,args", CASES, ids=[c[0] for c in CASES])
@@ -43,6 +53,7 @@ def test_element_assembly_matches_oracle(torch_gpu, name, builder, args):
     torch = torch_gpu
     print(f"[case] {name} {args}", flush=True)
     dofs, settings, static, o, prob = _setup(builder, *args)
+    assert prob.info.structured == STRUCTURED[name]
     rng = np.random.default_rng(7)
     u = rng.standard_normal(prob.n_dofs) * (0.02 if o["d"] > 1 else 1.0)
     ud = torch.from_numpy(u).cuda()
