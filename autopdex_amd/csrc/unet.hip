@@ -460,12 +460,42 @@ void restrict_(UNet& N, int level_fine, int C, const uint16_t* fine, uint16_t* o
 }
 }  // namespace
 
+int64_t UNet::conv_bytes(int li, int level) const {
+  const NetLayer& ly = layers[li];
+  return pixels(level) * (ly.ci * 2 + ly.co * (ly.out_f32 ? 4 : 2)) + (int64_t)ly.cop * ly.wk * 2;
+}
+
+int64_t UNet::forward_bytes() const {
+  const int L = levels;
+  int64_t b = conv_bytes(L_in, 0) + conv_bytes(L_b, L - 1) + conv_bytes(L_out, 0);
+  for (int l = 0; l + 1 < L; ++l) {
+    b += conv_bytes(L_e[l], l) + conv_bytes(L_down[l], l + 1) + conv_bytes(L_up[l], l + 1) + conv_bytes(L_d[l], l);
+    b += (pixels(l) + pixels(l + 1)) * ch[l] * 2;          // restriction
+    b += (2 * pixels(l) + pixels(l + 1)) * ch[l] * 2;      // prolongation + skip
+  }
+  return b;
+}
+
+void UNet::collect_timing() {
+  if (!timer_plan) return;
+  timing_collect(*timer_plan, "unet_conv_e0", ev_conv0, levels > 1 ? conv_bytes(L_e[0], 0) : 0, ev_conv0.size());
+  timing_collect(*timer_plan, "unet_forward", ev_fwd, forward_bytes(), ev_fwd.size());
+}
+
 void UNet::forward(hipStream_t st) {
   APX_REQUIRE(weights_ready, APX_ERR_STATE, "network weights not prepared");
   const int L = levels;
+  const bool timed = timer_plan && timer_plan->timing;
+  EventPair ef = timed ? timing_begin(*timer_plan, st) : EventPair{};
   conv_fwd(*this, L_in, 0, x0.p, a[0].p, st);
   for (int l = 0; l + 1 < L; ++l) {
+    EventPair ec{};
+    if (timed && l == 0) ec = timing_begin(*timer_plan, st);
     conv_fwd(*this, L_e[l], l, a[l].p, d[l].p, st);
+    if (timed && l == 0) {
+      timing_end(*timer_plan, ec, st);
+      ev_conv0.push_back(ec);
+    }
     restrict_(*this, l, ch[l], d[l].p, rd[l].p, st);
     conv_fwd(*this, L_down[l], l + 1, rd[l].p, a[l + 1].p, st);
   }
@@ -477,6 +507,10 @@ void UNet::forward(hipStream_t st) {
   }
   conv_fwd(*this, L_out, 0, u[0].p, y.p, st);
   APX_LAUNCH_CHECK();
+  if (timed) {
+    timing_end(*timer_plan, ef, st);
+    ev_fwd.push_back(ef);
+  }
 }
 
 double UNet::loss_backward(const float* target, hipStream_t st) {

@@ -38,6 +38,9 @@ struct UNet {
   DevBuf<double> loss_part;
   DevBuf<float> scale;             // per-image input scale
   bool weights_ready = false;
+  // launch timing (HIP events on the launch stream) when the attached plan times its kernels
+  struct Plan* timer_plan = nullptr;
+  std::vector<EventPair> ev_conv0, ev_fwd;
 
   void build(int levels, const int* channels, int c_in, int c_out, int relu, int batch, int H0, int W0);
   void prepare_weights(hipStream_t st);   // FP32 master -> bf16 forward / backward-data layouts
@@ -48,6 +51,12 @@ struct UNet {
   void backward(hipStream_t st);
   void adam(float lr, float b1, float b2, float eps, int step, hipStream_t st);
   int64_t pixels(int l) const { return (int64_t)batch * H[l] * W[l]; }
+  // algorithmic bytes: one convolution launch (input and output once, weights once),
+  // and one forward pass (every convolution and grid transfer)
+  int64_t conv_bytes(int layer, int level) const;
+  int64_t forward_bytes() const;
+  // move the recorded forward timings into the plan's timers
+  void collect_timing();
 };
 
 // Preconditioner use: z = s * net(r / s) on the plan's free DOFs (grid = nodes).

@@ -3,22 +3,31 @@
 Workload (one "step"): a full Newton solve on one GPU to a 1e-10 FP64
 relative residual ||r_free(u)||/||r_free(u0)||, inputs resident in HBM, for
 each method:
-  plain  = FP64 assembly + FP64 Jacobi-PCG (the drop-in method)
-  mixed  = FP32 tangent + one FP32 Jacobi-PCG re-anchored to the FP64
-           residual (reliable-update refinement)
+  plain         = FP64 assembly + FP64 Jacobi-PCG (the drop-in method)
+  mixed         = FP32 tangent + one FP32 Jacobi-PCG re-anchored to the FP64
+                  residual (reliable-update refinement)
+  learned       = FP64 tangent + flexible CG with the U-Net preconditioner
+                  (bf16 convolutions on MFMA, FP32 accumulation)
+  learned_mixed = FP32 tangent + FP32-matrix flexible CG with the U-Net, each
+                  inner solve re-anchored by the FP64 Newton residual
+The network is trained on the GPU before the timed steps (learned.fit: pairs
+from plain Jacobi-PCG solves of random right-hand sides, A-norm loss); its
+setup time is reported in `learned_setup` and is not part of any step.
 The headline `value` is the time-to-solution (seconds, lower is better) of
-the --headline method (default: the fastest method implemented, plain);
+the --headline method (default learned_mixed: learned + mixed precision);
 every method's time, Krylov iterations, residual and error against plain are
-reported beside it, with assembly throughput and a roofline for the dominant
-kernel (the fused PCG SpMV of the headline method) measured with HIP events on
-the launch stream during the timed steps.
+reported beside it, with assembly throughput and a roofline for the heaviest
+kernel of the headline method (the level-0 convolution for the learned
+methods, the fused PCG SpMV otherwise) measured with HIP events on the launch
+stream during the timed steps.
 
 The CPU baseline times the oracle (FP64 numpy/scipy restatement) on the box's
 host: residual + tangent assembly of a 1/16 element slab (scaled x16) plus a
 fixed number of scipy Jacobi-PCG iterations on the full-size reduced matrix
 (scaled to the plain method's iteration count).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--methods plain,mixed] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--methods plain,mixed,learned_mixed]
+                [--headline learned_mixed] [--train-steps 1500] [--no-cpu]
 """
 import argparse
 import ctypes as C
@@ -139,11 +148,11 @@ def cpu_baseline(prob, coords, elems, mask, plain_iters, vals64_cpu):
 def kernel_table(prob):
     out = {}
     for name in ("pcg_spmv_f64", "pcg_update_f64", "pcg_spmv_f32", "pcg_update_f32", "residual", "tangent_f64",
-                 "tangent_f32"):
+                 "tangent_f32", "fcg_spmv_f64", "fcg_spmv_f32", "fcg_precond", "unet_conv_e0", "unet_forward"):
         ms, cnt, by = timer(prob, name)
         if cnt:
-            out[name] = {"avg_ms": ms, "count": cnt, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9,
-                         "frac_hbm": by / (ms * 1e-3) / HBM_PEAK}
+            out[name] = {"avg_ms": ms, "count": cnt, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9 if by else None,
+                         "frac_hbm": by / (ms * 1e-3) / HBM_PEAK if by else None}
     return out
 
 
@@ -236,14 +245,20 @@ def main():
     T = {m: float(np.median(times[m])) for m in methods}
     kernels = kernel_table(prob)
     H = args.headline
-    prec_tag = "f64" if H == "plain" else "f32"
-    dom = kernels[f"pcg_spmv_{prec_tag}"]
-    dom_name = f"k_pcg_spmv<{'double' if prec_tag == 'f64' else 'float'}>"
+    if H.startswith("learned"):
+        # the heaviest single kernel of flexible CG with the network: the level-0 16 -> 16 convolution
+        dom = kernels["unet_conv_e0"]
+        dom_name = "k_conv3x3<16, 16, false, 1>"
+    else:
+        prec_tag = "f64" if H == "plain" else "f32"
+        dom = kernels[f"pcg_spmv_{prec_tag}"]
+        dom_name = f"k_pcg_spmv<{'double' if prec_tag == 'f64' else 'float'}>"
     res = kernels.get("residual", {})
+    dtypes = {"plain": "f64", "mixed": "f32+f64", "learned": "bf16+f64", "learned_mixed": "bf16+f32+f64"}
     out = {
         "metric": METRIC, "value": T[H], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": T[H] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if H == "plain" else "f32+f64", "data": "synthetic",
+        "dtype": dtypes[H], "data": "synthetic",
         "config": {"workload": f"2D Poisson Q1 {args.n}x{args.n} (short_example source, psdf Dirichlet), "
                                f"Newton to 1e-10 FP64 relative residual, method '{H}'",
                    "n_dofs": int(prob.n_dofs), "n_free": int(prob.n_free), "nnz": int(prob.nnz),
@@ -260,6 +275,8 @@ def main():
                      "tangent_f64_ms": kernels.get("tangent_f64", {}).get("avg_ms"),
                      "tangent_f32_ms": kernels.get("tangent_f32", {}).get("avg_ms")},
         "kernels": kernels,
+        "network": ({"forward_ms": kernels["unet_forward"]["avg_ms"], "forward_GBps": kernels["unet_forward"]["GBps"],
+                     "forward_bytes": kernels["unet_forward"]["bytes"]} if "unet_forward" in kernels else None),
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": dom["frac_hbm"], "traffic": PMC_TRAFFIC_BYTES.get((dom_name, args.n)),
                      "algorithmic_bytes_per_launch": dom["bytes"], "avg_ms": dom["avg_ms"]},
