@@ -90,6 +90,19 @@ struct DevBuf {
 };
 
 // ---- wave / block reductions (deterministic: fixed shuffle tree) ---------------
+// XCD-aware workgroup order.  MI355X deals the workgroups of a launch
+// round-robin to its 8 XCDs, each with its own L2.  Renumbering block b as
+// (b % 8) * (G / 8) + b / 8 gives every XCD one contiguous range of logical
+// blocks, so work whose inputs overlap (neighbouring SELL chunks gathering
+// the same vector entries, vertically adjacent tiles sharing halo rows) meets
+// in one L2.  G must be a multiple of 8 for the renumbering (otherwise the
+// identity is used).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
+  return (G % kXcds) ? b : (b % kXcds) * (G / kXcds) + b / kXcds;
+}
+__host__ inline int64_t round_xcd(int64_t g) { return (g + kXcds - 1) / kXcds * kXcds; }
+
 template <class T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

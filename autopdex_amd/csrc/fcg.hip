@@ -27,7 +27,7 @@ constexpr int kSpmvBS = 512;
 constexpr int kMaxBlocks = 8192;  // must match the partial-buffer layout of the Krylov kernels
 
 int grid_vec(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 1023) / 1024, kMaxBlocks)); }
-int grid_chunks(int64_t c) { return (int)std::max<int64_t>(1, std::min<int64_t>((c + 7) / 8, kMaxBlocks)); }
+int grid_chunks(int64_t c) { return (int)std::min<int64_t>(round_xcd(std::max<int64_t>(1, (c + 7) / 8)), kMaxBlocks); }
 
 template <int BS, int NV>
 __device__ inline void partials_out(double (&v)[NV], double* partials) {

@@ -59,7 +59,7 @@ static inline int grid_for(int64_t n) {
 }
 static inline int grid_chunks(int64_t n_chunks) {
   constexpr int wpb = kSpmvBS / 64;
-  return (int)std::max<int64_t>(1, std::min<int64_t>((n_chunks + wpb - 1) / wpb, kMaxBlocks));
+  return (int)std::min<int64_t>(round_xcd(std::max<int64_t>(1, (n_chunks + wpb - 1) / wpb)), kMaxBlocks);
 }
 
 void ensure_krylov_scratch(Plan& P, hipStream_t st) {

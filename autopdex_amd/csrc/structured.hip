@@ -51,6 +51,7 @@ struct TileArgs {
   void* vals;            // SELL values (tangent)
   int overwrite;         // 1: r = sum, 0: r += sum
   int want_source;       // residual: evaluate the u-independent part in the kernel
+  int64_t tiles, tiles_j;  // tiles in the grid, tiles per tile row
 };
 
 template <class M, class T, int MODE>
@@ -65,7 +66,11 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   constexpr int NR = (MODE == TILE_RES) ? kNE * NPE * D : kNE * NPE * NPE;
   __shared__ T er[NR];  // residual rows er[el][a][c]; element matrices er[el][a][b]
 
-  const int64_t i0 = (int64_t)blockIdx.y * kTI, j0 = (int64_t)blockIdx.x * kTJ;
+  // XCD-aware tile order (1D grid rounded up to a multiple of 8): consecutive logical tiles,
+  // row-major over the tile grid, share an L2
+  const int64_t lin = xcd_block(blockIdx.x, gridDim.x);
+  if (lin >= A.tiles) return;  // whole workgroup: no barrier is left waiting
+  const int64_t i0 = (lin / A.tiles_j) * kTI, j0 = (lin % A.tiles_j) * kTJ;
   const int64_t nx = A.nx, ny = A.ny;
   const int tid = threadIdx.x;
   // stage the 9 x 65 nodes around the tile (rows of 65 consecutive nodes)
@@ -228,9 +233,11 @@ void run_tile(Plan& P, ElemSet& S, const double* u, double* r, void* vals, const
     APX_REQUIRE(S.params.nodal[0] && S.params.nodal[1], APX_ERR_STATE,
                 "variable-coefficient Poisson set needs nodal fields g and f bound");
   }
-  const dim3 grid((unsigned)((S.gny + 1 + kTJ - 1) / kTJ), (unsigned)((S.gnx + 1 + kTI - 1) / kTI));
-  APX_REQUIRE(grid.y < 65536, APX_ERR_UNSUPPORTED, "structured grid too large for the owner-tile launch");
-  hipLaunchKernelGGL((k_tile<M, T, MODE>), grid, dim3(kTBS), 0, st, A);
+  A.tiles_j = (S.gny + 1 + kTJ - 1) / kTJ;
+  A.tiles = A.tiles_j * ((S.gnx + 1 + kTI - 1) / kTI);
+  const int64_t g = round_xcd(A.tiles);
+  APX_REQUIRE(g < ((int64_t)1 << 31), APX_ERR_UNSUPPORTED, "structured grid too large for the owner-tile launch");
+  hipLaunchKernelGGL((k_tile<M, T, MODE>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
   APX_LAUNCH_CHECK();
 }
 }  // namespace

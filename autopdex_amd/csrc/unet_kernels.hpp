@@ -18,6 +18,8 @@
 
 #include <cstdint>
 
+#include "common.hpp"
+
 namespace apx {
 namespace nn {
 
@@ -157,8 +159,9 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
       qs[r] = q;
     }
   };
-  if ((int64_t)blockIdx.x < ntiles) load_tile(blockIdx.x);
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
+  if (tile0 < ntiles) load_tile(tile0);
+  for (int64_t tile = tile0; tile < ntiles; tile += gridDim.x) {
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
     const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;

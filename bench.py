@@ -158,8 +158,8 @@ def kernel_table(prob):
 
 def probe(args):
     """Bounded kernel probe for hardware-counter runs: 3 residual and 3 tangent
-    assemblies, 20 fixed PCG iterations per precision, and a 512 MiB device copy
-    (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+    assemblies, 20 fixed PCG iterations per precision, 3 network applications,
+    and a 512 MiB device copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
     import torch
     from autopdex_amd import _native as N
     from autopdex_amd.backend import current_stream
@@ -179,6 +179,15 @@ def probe(args):
         ks = N.KrylovStats()
         N.call("apx_pcg", prob.plan, C.c_void_p(vals.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
                prec, N.APX_PRECOND_JACOBI, 0.0, 0.0, 20, C.byref(ks), st)
+    # three preconditioner applications of an (untrained) network of the benchmark's shape
+    from autopdex_amd import learned
+    H = args.n + 1
+    L = learned.default_levels(H, H)
+    net = learned.UNetPreconditioner(H, H, L, ((16, 32) + (64,) * 10)[:L], 1, 1, relu=False, batch=1)
+    net.attach(prob)
+    rb = (-r[r2f]).contiguous()
+    for _ in range(3):
+        net.apply(prob, rb)
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -196,6 +205,8 @@ def main():
     ap.add_argument("--headline", default="learned_mixed")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--seed", type=int, default=0, help="seed of the preconditioner's training data")
+    ap.add_argument("--save-net", default=None, help="write the trained network (.npz)")
+    ap.add_argument("--load-net", default=None, help="use a network written by --save-net instead of training")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="bounded kernel probe for counter runs")
     args = ap.parse_args()
@@ -219,8 +230,17 @@ def main():
     learned_info = None
     if any(m.startswith("learned") for m in methods):
         from autopdex_amd import learned
-        net, learned_info = learned.fit(prob, args.n + 1, args.n + 1, steps=args.train_steps, seed=args.seed,
-                                        log=lambda m: print(m, file=sys.stderr, flush=True))
+        if args.load_net:
+            t0 = time.perf_counter()
+            net = learned.UNetPreconditioner.load(args.load_net)
+            net.attach(prob)
+            learned_info = {"loaded": args.load_net, "load_seconds": time.perf_counter() - t0, "levels": net.levels,
+                            "channels": list(net.channels), "n_params": int(net.n_params)}
+        else:
+            net, learned_info = learned.fit(prob, args.n + 1, args.n + 1, steps=args.train_steps, seed=args.seed,
+                                            log=lambda m: print(m, file=sys.stderr, flush=True))
+        if args.save_net:
+            net.save(args.save_net)
         print(f"[bench] learned preconditioner: {learned_info}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         for m in methods:
