@@ -46,7 +46,8 @@ HBM_PEAK = 8.0e12
 # HBM bytes per executing launch at n = 2048, from `rocprofv3 --pmc FETCH_SIZE` and
 # `--pmc WRITE_SIZE` runs of `python bench.py --probe`, scaled by the counters'
 # reading of a 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000).
-PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double>", 2048): 674.2e6, ("k_pcg_spmv<float>", 2048): 409.2e6}
+PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double>", 2048): 674.2e6, ("k_pcg_spmv<float>", 2048): 409.6e6,
+                     ("k_conv3x3<16, 16, false, 1>", 2048): 287.8e6}
 
 
 def build_problem(n):
