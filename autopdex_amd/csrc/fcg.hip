@@ -80,14 +80,14 @@ __global__ __launch_bounds__(kFinBS) void k_fcg_init_fin(KState* S, const double
   }
 }
 
-// z = scale * y (grid -> free DOFs) and partials of z . q_prev
-__global__ void k_fcg_z(int64_t n, const int32_t* red2full, const float* y, const float* scale, const double* q,
-                        double* z, KState* S, double* partials) {
+// z = scale * dsq * y (grid -> free DOFs) and partials of z . q_prev
+__global__ void k_fcg_z(int64_t n, const int32_t* red2full, const float* y, const double* dsq, const float* scale,
+                        const double* q, double* z, KState* S, double* partials) {
   if (S->done) return;
   const float s = scale[0];
   double v[1] = {0.0};
   for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    const double zi = (double)(s * y[red2full[i]]);
+    const double zi = (double)(s * y[red2full[i]]) * (dsq ? dsq[i] : 1.0);
     z[i] = zi;
     v[0] += zi * q[i];
   }
@@ -196,6 +196,7 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   KState* S = (KState*)P.kstate.p;
   if (maxiter <= 0) maxiter = 10000;
   const int gv = grid_vec(n), gs = grid_chunks(P.n_chunks);
+  net_set_diag_scaling(P, vals, sizeof(T) == 8 ? APX_FP64 : APX_FP32, st);  // network sees D^-1/2 A D^-1/2
   APX_HIP(hipMemsetAsync(q, 0, n * sizeof(double), st));
   hipLaunchKernelGGL(k_fcg_init, dim3(gv), dim3(kBS), 0, st, n, b, x, r, pa, P.partials.p);
   hipLaunchKernelGGL(k_fcg_init_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv, tol, atol, maxiter);
@@ -206,7 +207,8 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
     EventPair e0 = timing_begin(P, st);
     net_input_from_reduced_img(P, N, r, 1, g_nodal, &S->rr, 0, st);
     N.forward(st);
-    hipLaunchKernelGGL(k_fcg_z, dim3(gv), dim3(kBS), 0, st, n, P.red2full.p, N.y.p, N.scale.p, q, z, S, P.partials.p);
+    hipLaunchKernelGGL(k_fcg_z, dim3(gv), dim3(kBS), 0, st, n, P.red2full.p, N.y.p, P.net_dsq.p, N.scale.p, q, z, S,
+                       P.partials.p);
     timing_end(P, e0, st);
     hipLaunchKernelGGL(k_fcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv);
     EventPair e1 = timing_begin(P, st);

@@ -124,7 +124,7 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
   NewtonTest test;
   double res = res0;
   bool have_tangent = false, go = true;
-  bool learned = o.precond == APX_PRECOND_LEARNED;
+  bool learned = o.precond == APX_PRECOND_LEARNED, fell_back = false;
   PcgRun<T> pcg(P, vals, inv, C.st);
   DevBuf<double> b64, x64;
   if (learned) {
@@ -153,11 +153,16 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
         // the network stopped reducing the residual: keep its progress, continue with Jacobi-PCG
         s.fallbacks += 1;
         learned = false;
+        fell_back = true;
       }
       scatter_update(P, C.u, x64.p, o.damping, APX_FP64, C.st);
     } else {
       gather_rhs(P, C.r, b, prec, C.st);
-      pcg.init(b, nullptr, eta, o.lin_atol, o.lin_maxiter, 0.0);
+      // after a fallback the iteration budget is Jacobi-PCG's own (at least jax cg's default
+      // 10 n_free): 'linear maxiter' was sized for flexible CG with the network
+      const int fb_max = (int)std::min<int64_t>(10 * P.n_free, 2000000000);
+      const int maxit = fell_back && o.lin_maxiter > 0 ? std::max(o.lin_maxiter, fb_max) : o.lin_maxiter;
+      pcg.init(b, nullptr, eta, o.lin_atol, maxit, 0.0);
       pcg.iterations = 0;
       pcg.counted = 0;
       KState hs = pcg.run();

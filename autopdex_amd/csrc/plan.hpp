@@ -83,6 +83,11 @@ struct Plan {
   double* pinned = nullptr;    // host pinned scalars
   struct UNet* net = nullptr;  // attached learned preconditioner (not owned)
   const double* net_g = nullptr;  // nodal coefficient channel of the network input (device, optional)
+  // Jacobi scaling of the network (1 / sqrt|A_ii| per free DOF): the network maps
+  // D^-1/2 r to D^1/2 z, i.e. it approximates the inverse of D^-1/2 A D^-1/2.
+  // Set from the matrix of the solve or training step that uses the network.
+  DevBuf<double> net_dsq;
+  bool net_dsq_set = false;
   bool timing = false;
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -571,11 +571,11 @@ void UNet::adam(float lr, float b1, float b2, float eps, int step, hipStream_t s
 // ---- reduced vector <-> node grid ----------------------------------------------------------------------
 namespace {
 template <class T>
-__global__ void k_sq_partials(int64_t n, const T* r, double* part) {
+__global__ void k_sq_partials(int64_t n, const T* r, const double* dsq, double* part) {
   __shared__ double lds[4];
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double v = (double)r[i];
+    const double v = dsq ? (double)r[i] * dsq[i] : (double)r[i];
     s += v * v;
   }
   s = block_sum<256>(s, lds);
@@ -599,29 +599,57 @@ __global__ void k_scale_fin(const double* part, int nb, const double* rr_src, in
 
 template <class T>
 __global__ void k_to_grid(int64_t n_nodes, int d, int cin, const uint8_t* dmask, const int32_t* full2red, const T* r,
-                          const double* g, const float* scale, int img, uint16_t* x) {
+                          const double* dsq, const double* g, const float* scale, int img, uint16_t* x) {
   const float inv = 1.f / scale[img];
   for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < n_nodes;
        node += (int64_t)gridDim.x * blockDim.x) {
     uint16_t* xo = x + ((int64_t)img * n_nodes + node) * cin;
     for (int c = 0; c < d; ++c) {
       const int64_t gd = node * d + c;
-      xo[c] = dmask[gd] ? (uint16_t)0 : f2bf((float)r[full2red[gd]] * inv);
+      if (dmask[gd]) {
+        xo[c] = 0;
+      } else {
+        const int32_t i = full2red[gd];
+        xo[c] = f2bf((float)((double)r[i] * (dsq ? dsq[i] : 1.0)) * inv);
+      }
     }
     if (cin > d) xo[d] = f2bf(g ? (float)g[node] : 0.f);
   }
 }
 
 template <class T>
-__global__ void k_from_grid(int64_t n_free, int d, const int32_t* red2full, const float* y, const float* scale,
-                            int img, int64_t n_nodes, T* z) {
+__global__ void k_from_grid(int64_t n_free, int d, const int32_t* red2full, const float* y, const double* dsq,
+                            const float* scale, int img, int64_t n_nodes, T* z) {
   const float s = scale[img];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_free; i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t gd = red2full[i];
-    z[i] = (T)(s * y[(int64_t)img * n_nodes * d + gd]);
+    const double v = (double)(s * y[(int64_t)img * n_nodes * d + gd]);
+    z[i] = (T)(dsq ? v * dsq[i] : v);
+  }
+}
+
+template <class T>
+__global__ void k_dsq(int64_t n, const int32_t* diag_slot, const T* vals, double* dsq) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = fabs((double)vals[diag_slot[i]]);
+    dsq[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
   }
 }
 }  // namespace
+
+void net_set_diag_scaling(Plan& P, const void* vals_sell, int precision, hipStream_t st) {
+  if (P.net_dsq.n < (size_t)P.n_free) P.net_dsq.alloc(P.n_free);
+  int g;
+  grid1d(P.n_free, g);
+  if (precision == APX_FP64)
+    hipLaunchKernelGGL(k_dsq<double>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const double*)vals_sell,
+                       P.net_dsq.p);
+  else
+    hipLaunchKernelGGL(k_dsq<float>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const float*)vals_sell,
+                       P.net_dsq.p);
+  APX_LAUNCH_CHECK();
+  P.net_dsq_set = true;
+}
 
 void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, const double* g_nodal,
                                 const double* rr_src, int img, hipStream_t st) {
@@ -629,33 +657,43 @@ void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, cons
   APX_REQUIRE(N.c_in == P.d || N.c_in == P.d + 1, APX_ERR_ARG, "network input channels do not match the DOFs");
   APX_REQUIRE(img >= 0 && img < N.batch, APX_ERR_ARG, "image index out of range");
   ensure_krylov_scratch(P, st);
+  const double* dsq = P.net_dsq_set ? P.net_dsq.p : nullptr;
+  if (dsq) rr_src = nullptr;  // the scale is the RMS of the scaled input
   int nb = 1;
   if (!rr_src) {
     grid1d(P.n_free, nb);
     nb = std::min(nb, 1024);
     if (r_f64)
-      hipLaunchKernelGGL(k_sq_partials<double>, dim3(nb), dim3(256), 0, st, P.n_free, (const double*)r, P.partials.p);
+      hipLaunchKernelGGL(k_sq_partials<double>, dim3(nb), dim3(256), 0, st, P.n_free, (const double*)r, dsq,
+                         P.partials.p);
     else
-      hipLaunchKernelGGL(k_sq_partials<float>, dim3(nb), dim3(256), 0, st, P.n_free, (const float*)r, P.partials.p);
+      hipLaunchKernelGGL(k_sq_partials<float>, dim3(nb), dim3(256), 0, st, P.n_free, (const float*)r, dsq,
+                         P.partials.p);
   }
   hipLaunchKernelGGL(k_scale_fin, dim3(1), dim3(256), 0, st, P.partials.p, nb, rr_src, P.n_free, N.scale.p, img);
   int g;
   grid1d(P.n_nodes, g);
   if (r_f64)
     hipLaunchKernelGGL(k_to_grid<double>, dim3(g), dim3(256), 0, st, P.n_nodes, P.d, N.c_in, P.dmask.p, P.full2red.p,
-                       (const double*)r, g_nodal, N.scale.p, img, N.x0.p);
+                       (const double*)r, dsq, g_nodal, N.scale.p, img, N.x0.p);
   else
     hipLaunchKernelGGL(k_to_grid<float>, dim3(g), dim3(256), 0, st, P.n_nodes, P.d, N.c_in, P.dmask.p, P.full2red.p,
-                       (const float*)r, g_nodal, N.scale.p, img, N.x0.p);
+                       (const float*)r, dsq, g_nodal, N.scale.p, img, N.x0.p);
   APX_LAUNCH_CHECK();
 }
 
 namespace {
 __global__ void k_target_grid(int64_t n_dofs, const uint8_t* dmask, const int32_t* full2red, const double* e,
-                              const float* scale, int img, float* t) {
+                              const double* dsq, const float* scale, int img, float* t) {
   const float inv = 1.f / scale[img];
-  for (int64_t gd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gd < n_dofs; gd += (int64_t)gridDim.x * blockDim.x)
-    t[(int64_t)img * n_dofs + gd] = dmask[gd] ? 0.f : (float)e[full2red[gd]] * inv;
+  for (int64_t gd = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gd < n_dofs; gd += (int64_t)gridDim.x * blockDim.x) {
+    if (dmask[gd]) {
+      t[(int64_t)img * n_dofs + gd] = 0.f;
+    } else {
+      const int32_t i = full2red[gd];
+      t[(int64_t)img * n_dofs + gd] = (float)(e[i] / (dsq ? dsq[i] : 1.0)) * inv;
+    }
+  }
 }
 }  // namespace
 
@@ -664,8 +702,8 @@ void net_target_from_reduced_img(Plan& P, UNet& N, const double* e, float* targe
   APX_REQUIRE(img >= 0 && img < N.batch, APX_ERR_ARG, "image index out of range");
   int g;
   grid1d(P.n_dofs, g);
-  hipLaunchKernelGGL(k_target_grid, dim3(g), dim3(256), 0, st, P.n_dofs, P.dmask.p, P.full2red.p, e, N.scale.p, img,
-                     target);
+  hipLaunchKernelGGL(k_target_grid, dim3(g), dim3(256), 0, st, P.n_dofs, P.dmask.p, P.full2red.p, e,
+                     P.net_dsq_set ? P.net_dsq.p : nullptr, N.scale.p, img, target);
   APX_LAUNCH_CHECK();
 }
 
@@ -673,12 +711,13 @@ void net_output_to_reduced_img(Plan& P, UNet& N, void* z, int z_f64, int img, hi
   APX_REQUIRE(N.c_out == P.d, APX_ERR_ARG, "network output channels do not match the DOFs");
   int g;
   grid1d(P.n_free, g);
+  const double* dsq = P.net_dsq_set ? P.net_dsq.p : nullptr;
   if (z_f64)
-    hipLaunchKernelGGL(k_from_grid<double>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, N.scale.p,
-                       img, P.n_nodes, (double*)z);
+    hipLaunchKernelGGL(k_from_grid<double>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, dsq,
+                       N.scale.p, img, P.n_nodes, (double*)z);
   else
-    hipLaunchKernelGGL(k_from_grid<float>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, N.scale.p,
-                       img, P.n_nodes, (float*)z);
+    hipLaunchKernelGGL(k_from_grid<float>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, dsq,
+                       N.scale.p, img, P.n_nodes, (float*)z);
   APX_LAUNCH_CHECK();
 }
 
@@ -732,13 +771,15 @@ __global__ void k_dots3(int64_t n, const double* a0, const double* b0, const dou
   }
 }
 
+// dL/dy at the network output: z = s * dsq * y, so dL/dy = s * dsq * dL/dz
 __global__ void k_grad_out(int64_t n, const int32_t* red2full, const double* Ares, const double* z, const double* e,
-                           double cr, double ce, const float* scale, int img, float inv_batch, int64_t n_dofs,
-                           uint16_t* gy) {
+                           double cr, double ce, const double* dsq, const float* scale, int img, float inv_batch,
+                           int64_t n_dofs, uint16_t* gy) {
   const double s = scale[img];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double g = -cr * Ares[i];
     if (e) g += ce * (z[i] - e[i]);
+    if (dsq) g *= dsq[i];
     gy[(int64_t)img * n_dofs + red2full[i]] = f2bf((float)(s * g * inv_batch));
   }
 }
@@ -755,6 +796,7 @@ double unet_residual_loss_backward(Plan& P, UNet& N, const double* vals_sell, co
   grid1d(n, g);
   const int gr = std::min(g, kRedBlocks);
   APX_HIP(hipMemsetAsync(N.gy.p, 0, N.gy.bytes(), st));
+  net_set_diag_scaling(P, vals_sell, APX_FP64, st);  // the network sees D^-1/2 A D^-1/2
   // sign of the (definite) matrix from its first diagonal entry
   double d0 = 1.0;
   {
@@ -806,13 +848,13 @@ double unet_residual_loss_backward(Plan& P, UNet& N, const double* vals_sell, co
       }
       // g = -cr res (+ the error term) through the Ares slot of k_grad_out
       hipLaunchKernelGGL(k_grad_out, dim3(g), dim3(256), 0, st, n, P.red2full.p, res, z, eb, cr, 2.0 * w_e / ee,
-                         N.scale.p, b, 1.f / N.batch, P.n_dofs, N.gy.p);
+                         P.net_dsq.p, N.scale.p, b, 1.f / N.batch, P.n_dofs, N.gy.p);
     } else {
       loss += t[0] / rr + (eb ? w_e * t[2] / ee : 0.0);
       spmv(P, vals_sell, res, Ares, APX_FP64, st);
       // d/dz: -2 A (r - A z) / ||r||^2 + 2 w_e (z - e) / ||e||^2 ; d/dy = s d/dz
       hipLaunchKernelGGL(k_grad_out, dim3(g), dim3(256), 0, st, n, P.red2full.p, Ares, z, eb, 2.0 / rr,
-                         2.0 * w_e / ee, N.scale.p, b, 1.f / N.batch, P.n_dofs, N.gy.p);
+                         2.0 * w_e / ee, P.net_dsq.p, N.scale.p, b, 1.f / N.batch, P.n_dofs, N.gy.p);
     }
     APX_LAUNCH_CHECK();
   }

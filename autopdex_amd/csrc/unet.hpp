@@ -59,7 +59,11 @@ struct UNet {
   void collect_timing();
 };
 
-// Preconditioner use: z = s * net(r / s) on the plan's free DOFs (grid = nodes).
+// Network scaling from the diagonal of a SELL matrix (values in `precision`).
+void net_set_diag_scaling(Plan& P, const void* vals_sell, int precision, hipStream_t st);
+
+// Preconditioner use: z = s * net(r / s) on the plan's free DOFs (grid = nodes),
+// with r and z scaled by D^-1/2 when the plan's diagonal scaling is set.
 // rr_src (device, optional) holds sum r^2 over the free DOFs; s = sqrt(rr / n_free).
 void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, const double* g_nodal,
                                 const double* rr_src, int img, hipStream_t st);

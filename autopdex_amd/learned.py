@@ -227,8 +227,8 @@ def default_levels(height, width, coarsest=3):
 def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=16, n_smooth=16,
         pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
         log=None, time_budget=None):
-    """Train a linear U-Net preconditioner for `problem` (scalar field on a structured
-    height x width node grid) and attach it.
+    """Train a linear U-Net preconditioner for `problem` (d DOFs per node on a structured
+    height x width node grid: d input and output channels) and attach it.
 
     Training pairs come from the plain solver: Jacobi-PCG solves of A e = r for white-noise
     and smooth random right-hand sides r (squared-exponential fields of lengths 0.2 ... 0.005).
@@ -249,8 +249,12 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     def noise():
         return torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64)
 
+    d = problem.n_dofs // (height * width)  # DOFs per node (fields interleaved node-major, P3)
+
     def smooth(s):
-        return random_field(height, width, lengths[s % len(lengths)], seed=seed * 1000 + s)[free].contiguous()
+        comps = [random_field(height, width, lengths[s % len(lengths)], seed=seed * 1000 + d * s + c)
+                 for c in range(d)]
+        return torch.stack(comps, dim=1).reshape(-1)[free].contiguous()
 
     rhs = [noise() for _ in range(n_noise)] + [smooth(s) for s in range(n_smooth)]
     pairs = []
@@ -262,9 +266,9 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     if log:
         log(f"[fit] {len(pairs)} (r, A^-1 r) pairs from {len(rhs)} plain solves in {t_pairs:.1f} s; levels {L}, "
             f"channels {ch}")
-    net = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=batch, seed=seed + 1)
+    net = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=batch, seed=seed + 1)
     net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
-    inf = UNetPreconditioner(height, width, L, ch, 1, 1, relu=False, batch=1)
+    inf = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=1)
     inf.attach(problem)
     t1 = time.perf_counter()
     best, history, done = None, [], 0
