@@ -300,6 +300,97 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     return inf, info
 
 
+def fit_fields(problem, height, width, fields, set_index=0, levels=None, channels=None, n_noise=2, n_smooth=2,
+               pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
+               validation_field=None, log=None):
+    """Train one network for a family of coefficient fields of a variable-coefficient model
+    (models.poisson_variable_coefficient_integrand): the nodal log-coefficient g is the
+    network's extra input channel, so it can follow the field it is applied to.
+
+    fields: list of (g, f) device tensors (FP64, one value per node).  For each field the
+    plan's tangent is assembled and the plain solver (Jacobi-PCG) produces (r, A^-1 r) pairs for
+    noise and smooth right-hand sides.  Every training step draws one field and a batch of its
+    pairs, so the A-norm loss uses that field's matrix.  validation_field (g, f), held out,
+    scores the parameters by flexible-CG iterations (to 1e-6) every `validate_every` steps; the
+    best are kept.  Returns (network with batch 1, attached with the validation field's g when
+    given, else the last field's; info dict).  Leaves the last bound field on the plan."""
+    torch = torch_cuda()
+    t0 = time.perf_counter()
+    L = levels or default_levels(height, width)
+    ch = tuple(channels or ((16, 32) + (64,) * 10)[:L])
+    free = torch.from_numpy(problem.free_dofs().astype(np.int64)).to(problem.device)
+    gen = torch.Generator(device=problem.device)
+    gen.manual_seed(int(seed))
+    lengths = (0.2, 0.05, 0.02, 0.005)
+    data = []
+    for fi, (g, f) in enumerate(fields):
+        problem.bind_nodal_device(set_index, g, f)
+        vals = tangent(problem)
+        rhs = [torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64)
+               for _ in range(n_noise)]
+        rhs += [random_field(height, width, lengths[(fi + s) % len(lengths)], seed=seed * 100003 + fi * 101 + s)[free]
+                .contiguous() for s in range(n_smooth)]
+        prs = []
+        for b in rhs:
+            prs += solver_pairs(problem, vals, b, checkpoints=(0,), tol=pair_tol)
+        data.append((vals, g, prs))
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    if log:
+        log(f"[fit_fields] {sum(len(d[2]) for d in data)} pairs on {len(data)} fields in {t_pairs:.1f} s; "
+            f"levels {L}, channels {ch}")
+    d = problem.n_dofs // (height * width)
+    net = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=batch, seed=seed + 1)
+    net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
+    inf = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=1)
+    val = None
+    if validation_field is not None:
+        gv, fv = validation_field
+        problem.bind_nodal_device(set_index, gv, fv)
+        val = (tangent(problem), gv,
+               torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64))
+    rng = np.random.default_rng(seed)
+    B = batch
+    history, best, done = [], None, 0
+    t1 = time.perf_counter()
+    seg = validate_every if (validate_every and val is not None) else steps
+    while done < steps:
+        k = min(seg, steps - done)
+        for it in range(done, done + k):
+            vals, g, prs = data[rng.integers(len(data))]
+            idx = rng.integers(0, len(prs), size=B)
+            r_list = (C.c_void_p * B)(*[prs[i][0].data_ptr() for i in idx])
+            e_list = (C.c_void_p * B)(*[prs[i][1].data_ptr() for i in idx])
+            g_list = (C.c_void_p * B)(*([g.data_ptr()] * B))
+            lval = C.c_double()
+            _call("apx_unet_train_step", net.handle, problem.plan, _vp(vals), r_list, g_list, e_list,
+                  _LOSS_KINDS["anorm"], 0.0, C.byref(lval), current_stream())
+            warm = min(1.0, (it + 1) / 50)
+            net.adam(warm * lr * 0.1 ** (it / max(steps - 1, 1)))
+            history.append(lval.value)
+        done += k
+        theta = net.get_params()
+        if val is not None:
+            inf.set_params(theta)
+            inf.attach(problem, val[1])
+            _, ks = fcg(problem, val[0], val[2], tol=1e-6, maxiter=400)
+            score = ks.iterations if ks.converged else 10 ** 6
+            if log:
+                log(f"[fit_fields] step {done}: loss {np.mean(history[-50:]):.3e}, held-out field FCG {ks.iterations} it")
+        else:
+            score = 0
+        if best is None or score <= best[0]:
+            best = (score, theta, done)
+    torch.cuda.synchronize()
+    inf.set_params(best[1])
+    inf.attach(problem, val[1] if val is not None else data[-1][1])
+    info = dict(fields=len(data), pairs=sum(len(x[2]) for x in data), pair_seconds=t_pairs,
+                train_seconds=time.perf_counter() - t1, steps=done, best_step=best[2],
+                validation_iterations=best[0] if val is not None else None, levels=L, channels=list(ch),
+                final_loss=float(np.mean(history[-50:])), n_params=int(inf.n_params))
+    return inf, info
+
+
 def solver_pairs(problem, vals, rhs, checkpoints=(0, 4, 16, 48, 128, 320, 800, 2000), tol=1e-10):
     """(r_k, e_k) pairs from Jacobi-PCG iterates of A x = b: e_k = x* - x_k, r_k = b - A x_k.
     Restarted PCG runs with maxiter = k reproduce the iterates x_k of the full solve."""

@@ -6,7 +6,9 @@
   rounding-boundary effects only);
 - a network trained on the GPU on pairs from the plain solver needs fewer
   Krylov iterations than Jacobi-PCG and reaches the SuperLU solution to 1e-8;
-- a broken network triggers the Jacobi-PCG fallback and Newton still converges.
+- a broken network triggers the Jacobi-PCG fallback and Newton still converges;
+- a network trained on several coefficient fields of the variable-coefficient
+  model (coefficient as input channel) cuts iterations on held-out fields.
 """
 import ctypes as C
 
@@ -137,3 +139,48 @@ def test_learned_newton_through_solver_and_fallback(torch_gpu):
     sol, infos = solver.solver(dofs, settings, s, newton_tol=1e-10)
     assert infos[2] is False and infos.details["fallbacks"] == 1
     assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_coefficient_family_network_generalizes_to_held_out_fields(torch_gpu):
+    """Variable-coefficient Poisson (k = exp(g), g a squared-exponential random field): one
+    network with g as input channel, trained on pairs from 8 fields, cuts flexible-CG
+    iterations on fields it has not seen and reaches the SuperLU solution."""
+    torch = torch_gpu
+    from autopdex_amd import _native as N
+    from autopdex_amd import backend, learned
+    from autopdex_amd.backend import current_stream
+    from oracle import assembly as OA
+    n = 128
+    H = n + 1
+    dofs, settings, static, o = P.poisson_vark(n, seed=0)
+    prob = backend.get_problem(dofs, settings, static)
+
+    def field(seed):
+        g = learned.random_field(H, H, 0.05, 1.0, seed=seed).contiguous()
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed + 1_000_000)
+        return g, torch.randn(H * H, generator=gen, device="cuda", dtype=torch.float64)
+
+    net, info = learned.fit_fields(prob, H, H, [field(s) for s in range(8)], steps=1500,
+                                   validation_field=field(999), validate_every=250)
+    r2f = torch.from_numpy(prob.free_dofs().astype(np.int64)).cuda()
+    for s in (5000, 5001):
+        g, f = field(s)
+        prob.bind_nodal_device(0, g, f)
+        vals = learned.tangent(prob)
+        u0 = torch.zeros(prob.n_dofs, dtype=torch.float64, device="cuda")
+        r = torch.empty_like(u0)
+        N.call("apx_assemble_residual", prob.plan, C.c_void_p(u0.data_ptr()), C.c_void_p(r.data_ptr()),
+               current_stream())
+        b = (-r[r2f]).contiguous()
+        net.attach(prob, g)
+        x, kf = learned.fcg(prob, vals, b, tol=1e-10, maxiter=2000)
+        xp, kp = learned.pcg(prob, vals, b, tol=1e-10)
+        print(f"[held-out field {s}] FCG+net {kf.iterations} it vs Jacobi-PCG {kp.iterations} it", flush=True)
+        assert kf.converged and kf.iterations < 0.5 * kp.iterations
+        # the oracle's reduced matrix of this field solves to the same vector
+        A = OA.reduced_tangent([(o["sets"][0][0], o["sets"][0][1], (g.cpu().numpy(), f.cpu().numpy()))],
+                               o["coords"], np.zeros(prob.n_dofs), 1, ~o["mask"])
+        import scipy.sparse.linalg as spla
+        x_ref = spla.spsolve(A.tocsc(), b.cpu().numpy())
+        assert np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) < 1e-8
