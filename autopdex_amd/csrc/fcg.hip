@@ -120,9 +120,9 @@ __global__ __launch_bounds__(kSpmvBS) void k_fcg_spmv(int64_t n, int64_t n_chunk
     const T* vv = vals + off + lane;
     const int32_t* ci = cols + off + lane;
     double s = 0.0;
-    for (int k = 0; k < w; ++k) {
-      const int32_t col = ci[k * 64];
-      s += (double)vv[k * 64] * (z[col] + beta * po[col]);
+    for (int k = 0; k < w; ++k) {  // values and columns are read once: non-temporal
+      const int32_t col = __builtin_nontemporal_load(ci + k * 64);
+      s += (double)__builtin_nontemporal_load(vv + k * 64) * (z[col] + beta * po[col]);
     }
     const int64_t row = c * 64 + lane;
     if (row < n) {

@@ -107,14 +107,17 @@ __device__ __forceinline__ T sell_row(const T* __restrict__ v, const int32_t* __
   T s = 0;
   int k = 0;
   for (; k + 4 <= w; k += 4) {
-    const int32_t c0 = ci[(k + 0) * 64], c1 = ci[(k + 1) * 64], c2 = ci[(k + 2) * 64], c3 = ci[(k + 3) * 64];
-    const T v0 = v[(k + 0) * 64], v1 = v[(k + 1) * 64], v2 = v[(k + 2) * 64], v3 = v[(k + 3) * 64];
+    // values and columns are read once per SpMV: non-temporal, so L2 keeps the gathered vector
+    const int32_t c0 = __builtin_nontemporal_load(ci + (k + 0) * 64), c1 = __builtin_nontemporal_load(ci + (k + 1) * 64),
+                  c2 = __builtin_nontemporal_load(ci + (k + 2) * 64), c3 = __builtin_nontemporal_load(ci + (k + 3) * 64);
+    const T v0 = __builtin_nontemporal_load(v + (k + 0) * 64), v1 = __builtin_nontemporal_load(v + (k + 1) * 64),
+            v2 = __builtin_nontemporal_load(v + (k + 2) * 64), v3 = __builtin_nontemporal_load(v + (k + 3) * 64);
     s += v0 * f(c0);
     s += v1 * f(c1);
     s += v2 * f(c2);
     s += v3 * f(c3);
   }
-  for (; k < w; ++k) s += v[k * 64] * f(ci[k * 64]);
+  for (; k < w; ++k) s += __builtin_nontemporal_load(v + k * 64) * f(__builtin_nontemporal_load(ci + k * 64));
   return s;
 }
 
