@@ -202,7 +202,17 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   hipLaunchKernelGGL(k_fcg_init_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv, tol, atol, maxiter);
   APX_LAUNCH_CHECK();
   std::vector<EventPair> ev_net, ev_spmv;
-  N.timer_plan = P.timing ? &P : nullptr;  // the network times its forward passes of this solve
+  // for this solve only: the network times its forward passes, and iterations enqueued after
+  // convergence skip it (restored on every exit)
+  struct NetScope {
+    UNet& N;
+    ~NetScope() {
+      N.timer_plan = nullptr;
+      N.stop = nullptr;
+    }
+  } net_scope{N};
+  N.timer_plan = P.timing ? &P : nullptr;
+  N.stop = &S->done;
   auto iteration = [&] {
     EventPair e0 = timing_begin(P, st);
     net_input_from_reduced_img(P, N, r, 1, g_nodal, &S->rr, 0, st);
@@ -284,9 +294,8 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
     const int64_t spmv_bytes = P.sell_total * (int64_t)(sizeof(T) + 4) + (P.n_chunks + 1) * 4 + n * 6 * 8;
     timing_collect(P, "fcg_precond", ev_net, 0, k);
     timing_collect(P, sizeof(T) == 8 ? "fcg_spmv_f64" : "fcg_spmv_f32", ev_spmv, spmv_bytes, k);
-    N.collect_timing();
+    N.collect_timing(k);  // the forward passes of the executed iterations
   }
-  N.timer_plan = nullptr;
   return res;
 }
 

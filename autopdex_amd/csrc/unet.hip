@@ -111,9 +111,9 @@ __device__ __forceinline__ u32x4 bf8_pack(const float f[8]) {
 
 template <int C8>
 __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
-                                                     uint16_t* out, int H, int W, int Hc, int Wc) {
+                                                     uint16_t* out, int H, int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= W * C8) return;
+  if (idx >= W * C8 || (stop && *stop)) return;
   const int j = idx / C8, c = (idx % C8) * 8, i = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
   const int I0 = i >> 1, J0 = j >> 1;
@@ -146,9 +146,9 @@ __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict_
 // R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj).
 template <int C8>
 __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int H,
-                                                  int W, int Hc, int Wc) {
+                                                  int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= Wc * C8) return;
+  if (idx >= Wc * C8 || (stop && *stop)) return;
   const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
   const uint16_t* fb = fine + (size_t)b * H * W * C + c;
@@ -390,6 +390,7 @@ void conv_fwd(UNet& N, int li, int level, const uint16_t* x, void* y, hipStream_
   a.H = N.H[level];
   a.W = N.W[level];
   a.relu = ly.relu;
+  a.stop = N.stop;
   launch_conv(ly.ci, ly.co, ly.out_f32 != 0, a, st);
 }
 
@@ -435,11 +436,11 @@ void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const u
   const dim3 grid((W * (C / 8) + 255) / 256, H, N.batch);
   const int Hc = N.H[level_fine + 1], Wc = N.W[level_fine + 1];
   if (C == 16)
-    hipLaunchKernelGGL(k_prolong_add<2>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc);
+    hipLaunchKernelGGL(k_prolong_add<2>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else if (C == 32)
-    hipLaunchKernelGGL(k_prolong_add<4>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc);
+    hipLaunchKernelGGL(k_prolong_add<4>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else if (C == 64)
-    hipLaunchKernelGGL(k_prolong_add<8>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc);
+    hipLaunchKernelGGL(k_prolong_add<8>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else
     throw Error(APX_ERR_UNSUPPORTED, "grid transfer: channels must be 16, 32 or 64");
 }
@@ -450,11 +451,11 @@ void restrict_(UNet& N, int level_fine, int C, const uint16_t* fine, uint16_t* o
   APX_REQUIRE(C % 8 == 0 && Hc < 65536 && N.batch < 65536, APX_ERR_UNSUPPORTED, "grid transfer shape");
   const dim3 grid((Wc * (C / 8) + 255) / 256, Hc, N.batch);
   if (C == 16)
-    hipLaunchKernelGGL(k_restrict<2>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc);
+    hipLaunchKernelGGL(k_restrict<2>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
   else if (C == 32)
-    hipLaunchKernelGGL(k_restrict<4>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc);
+    hipLaunchKernelGGL(k_restrict<4>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
   else if (C == 64)
-    hipLaunchKernelGGL(k_restrict<8>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc);
+    hipLaunchKernelGGL(k_restrict<8>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
   else
     throw Error(APX_ERR_UNSUPPORTED, "grid transfer: channels must be 16, 32 or 64");
 }
@@ -476,10 +477,10 @@ int64_t UNet::forward_bytes() const {
   return b;
 }
 
-void UNet::collect_timing() {
+void UNet::collect_timing(size_t count) {
   if (!timer_plan) return;
-  timing_collect(*timer_plan, "unet_conv_e0", ev_conv0, levels > 1 ? conv_bytes(L_e[0], 0) : 0, ev_conv0.size());
-  timing_collect(*timer_plan, "unet_forward", ev_fwd, forward_bytes(), ev_fwd.size());
+  timing_collect(*timer_plan, "unet_conv_e0", ev_conv0, levels > 1 ? conv_bytes(L_e[0], 0) : 0, count);
+  timing_collect(*timer_plan, "unet_forward", ev_fwd, forward_bytes(), count);
 }
 
 void UNet::forward(hipStream_t st) {
@@ -571,8 +572,9 @@ void UNet::adam(float lr, float b1, float b2, float eps, int step, hipStream_t s
 // ---- reduced vector <-> node grid ----------------------------------------------------------------------
 namespace {
 template <class T>
-__global__ void k_sq_partials(int64_t n, const T* r, const double* dsq, double* part) {
+__global__ void k_sq_partials(int64_t n, const T* r, const double* dsq, double* part, const int* stop) {
   __shared__ double lds[4];
+  if (stop && *stop) return;
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double v = dsq ? (double)r[i] * dsq[i] : (double)r[i];
@@ -582,8 +584,10 @@ __global__ void k_sq_partials(int64_t n, const T* r, const double* dsq, double* 
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void k_scale_fin(const double* part, int nb, const double* rr_src, int64_t n, float* scale, int img) {
+__global__ void k_scale_fin(const double* part, int nb, const double* rr_src, int64_t n, float* scale, int img,
+                            const int* stop) {
   __shared__ double lds[4];
+  if (stop && *stop) return;
   double s = 0.0;
   if (rr_src) {
     s = *rr_src;
@@ -599,7 +603,9 @@ __global__ void k_scale_fin(const double* part, int nb, const double* rr_src, in
 
 template <class T>
 __global__ void k_to_grid(int64_t n_nodes, int d, int cin, const uint8_t* dmask, const int32_t* full2red, const T* r,
-                          const double* dsq, const double* g, const float* scale, int img, uint16_t* x) {
+                          const double* dsq, const double* g, const float* scale, int img, uint16_t* x,
+                          const int* stop) {
+  if (stop && *stop) return;
   const float inv = 1.f / scale[img];
   for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < n_nodes;
        node += (int64_t)gridDim.x * blockDim.x) {
@@ -665,20 +671,21 @@ void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, cons
     nb = std::min(nb, 1024);
     if (r_f64)
       hipLaunchKernelGGL(k_sq_partials<double>, dim3(nb), dim3(256), 0, st, P.n_free, (const double*)r, dsq,
-                         P.partials.p);
+                         P.partials.p, N.stop);
     else
       hipLaunchKernelGGL(k_sq_partials<float>, dim3(nb), dim3(256), 0, st, P.n_free, (const float*)r, dsq,
-                         P.partials.p);
+                         P.partials.p, N.stop);
   }
-  hipLaunchKernelGGL(k_scale_fin, dim3(1), dim3(256), 0, st, P.partials.p, nb, rr_src, P.n_free, N.scale.p, img);
+  hipLaunchKernelGGL(k_scale_fin, dim3(1), dim3(256), 0, st, P.partials.p, nb, rr_src, P.n_free, N.scale.p, img,
+                     N.stop);
   int g;
   grid1d(P.n_nodes, g);
   if (r_f64)
     hipLaunchKernelGGL(k_to_grid<double>, dim3(g), dim3(256), 0, st, P.n_nodes, P.d, N.c_in, P.dmask.p, P.full2red.p,
-                       (const double*)r, dsq, g_nodal, N.scale.p, img, N.x0.p);
+                       (const double*)r, dsq, g_nodal, N.scale.p, img, N.x0.p, N.stop);
   else
     hipLaunchKernelGGL(k_to_grid<float>, dim3(g), dim3(256), 0, st, P.n_nodes, P.d, N.c_in, P.dmask.p, P.full2red.p,
-                       (const float*)r, dsq, g_nodal, N.scale.p, img, N.x0.p);
+                       (const float*)r, dsq, g_nodal, N.scale.p, img, N.x0.p, N.stop);
   APX_LAUNCH_CHECK();
 }
 

@@ -41,6 +41,9 @@ struct UNet {
   // launch timing (HIP events on the launch stream) when the attached plan times its kernels
   struct Plan* timer_plan = nullptr;
   std::vector<EventPair> ev_conv0, ev_fwd;
+  // optional device stop flag (a Krylov solver's done flag): forward-pass kernels launched
+  // after it is set return at once
+  const int* stop = nullptr;
 
   void build(int levels, const int* channels, int c_in, int c_out, int relu, int batch, int H0, int W0);
   void prepare_weights(hipStream_t st);   // FP32 master -> bf16 forward / backward-data layouts
@@ -55,8 +58,8 @@ struct UNet {
   // and one forward pass (every convolution and grid transfer)
   int64_t conv_bytes(int layer, int level) const;
   int64_t forward_bytes() const;
-  // move the recorded forward timings into the plan's timers
-  void collect_timing();
+  // move the first `count` recorded forward timings into the plan's timers (the rest are dropped)
+  void collect_timing(size_t count);
 };
 
 // Network scaling from the diagonal of a SELL matrix (values in `precision`).

@@ -52,6 +52,7 @@ struct ConvArgs {
   const float* yscale;    // optional: per-image output scale (float y only)
   int B, H, W;
   int relu;
+  const int* stop;        // optional device flag: nonzero -> the launch does nothing
 };
 
 // ---- forward / backward-data convolution ------------------------------------
@@ -93,6 +94,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   static_assert(OUT_F32 || CB % 8 == 0, "bf16 outputs are stored in 8-channel chunks");
   __shared__ __attribute__((aligned(16))) uint16_t sbuf[SBUF];
   __shared__ __attribute__((aligned(16))) uint16_t sw[16 * NB * SWK];
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   uint16_t* sx = sbuf;
   const int co0 = blockIdx.y * 16 * NB;  // first output channel of this block
   // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; all
