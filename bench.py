@@ -214,8 +214,8 @@ def main():
     if args.probe:
         return probe(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the assemble-and-solve path runs on a single MI355X in this build "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the assemble-and-solve path runs on a single MI355X "
+              "(multi-GPU sharding is a design, not implemented). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import torch
     from autopdex_amd import _native as N
