@@ -12,6 +12,14 @@ TAPERED = [[0.0, 0.0], [48.0, 44.0], [48.0, 60.0], [0.0, 44.0]]
 UNIT_CUBE = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
 
 
+def golden(name):
+    """A known answer from tests/golden/reference_goldens.json (the reference's own test values)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")) as f:
+        return json.load(f)[name]["value"]
+
+
 def static(model_list, **extra):
     s = {
         "assembling mode": tuple("user potential" for _ in model_list),

@@ -133,7 +133,7 @@ def test_reference_known_answer_5x5(torch_gpu):
     from autopdex_amd import solver
     dofs, settings, static, o = P.poisson_source(5)
     sol, infos = solver.solver(dofs, settings, static)
-    assert np.isclose(sol["phi"].sum(), 1.9066412530282952, rtol=1e-10, atol=0)
+    assert np.isclose(sol["phi"].sum(), P.golden("short_example_5x5_sum_phi"), rtol=1e-10, atol=0)
     assert infos[2] is False and infos[1] <= 1e-8
 
 

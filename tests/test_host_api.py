@@ -78,3 +78,30 @@ def test_solver_rejects_other_backends_and_types():
     assert o64.reuse_tangent == 1 and o64.precision == 0 and abs(o64.lin_tol - 1e-5) < 1e-20
     dofs, settings, static, o = P.neo_hooke_cube(2)
     assert solver.newton_options(static).reuse_tangent == 0
+
+
+def test_random_field_statistics():
+    """learned.random_field: zero mean, unit variance, squared-exponential correlation
+    exp(-d^2 / (2 l^2)) at a lag d (averaged over seeds; CPU)."""
+    from autopdex_amd import learned
+    H, length = 65, 0.1
+    lag = 8  # nodes; d = lag / (H - 1)
+    d = lag / (H - 1)
+    vals, corr = [], []
+    for s in range(40):
+        f = learned.random_field(H, H, length, 1.0, seed=s, device="cpu").numpy().reshape(H, H)
+        vals.append(f.ravel())
+        corr.append(np.mean(f[:, :-lag] * f[:, lag:]))
+    v = np.concatenate(vals)
+    assert abs(v.mean()) < 0.1 and abs(v.var() - 1.0) < 0.15
+    assert abs(np.mean(corr) - np.exp(-d ** 2 / (2 * length ** 2))) < 0.1
+
+
+def test_default_levels_reach_a_3x3_grid():
+    from autopdex_amd import learned
+    assert learned.default_levels(2049, 2049) == 11   # 2049 -> 1025 -> ... -> 5 -> 3
+    assert learned.default_levels(1025, 1025) == 10
+    assert learned.default_levels(257, 257) == 8
+    assert learned.default_levels(33, 33) == 5
+    assert learned.default_levels(513, 129) == 7      # limited by the shorter side
+    assert learned.default_levels(3, 3) == 1

@@ -27,7 +27,7 @@ def test_reference_golden_5x5():
     sol, info = ON.newton(np.zeros(36), lambda u: OA.residual(sets, coords, u, 1),
                           lambda u: OA.reduced_tangent(sets, coords, u, 1, ~mask), mask, np.zeros(36),
                           lambda A, b: ON.linear_solve(A, b)[0])
-    assert abs(sol.sum() - 1.9066412530282952) < 1e-14
+    assert abs(sol.sum() - P.golden("short_example_5x5_sum_phi")) < 1e-14
     assert info[0] == 1 and info[2] is False
 
 
