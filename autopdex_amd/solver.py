@@ -111,6 +111,71 @@ def solver(dofs, settings, static_settings, **kwargs):
             raise NotImplementedError(f"solver type {solver_type!r} is not available in the 'hip' backend")
 
 
+def _load_multiplier(settings, multiplier):
+    settings.update({"load multiplier": multiplier})
+    return settings
+
+
+def adaptive_load_stepping(dofs, settings, static_settings, multiplier_settings=_load_multiplier,
+                           path_dependent=True, implicit_diff_mode=None, max_multiplier=1.0, min_increment=0.01,
+                           max_increment=1.0, init_increment=0.2, max_load_steps=1000, target_num_newton_iter=7,
+                           newton_tol=1e-10, **kwargs):
+    """solver.adaptive_load_stepping (solver.py:139-457) without implicit differentiation.
+
+    The increment control is the reference's (:296-379): multiplier += increment, one Newton
+    solve; on divergence the step is undone and the increment halved, otherwise the increment
+    is scaled by 1 + 0.5 (target - n_iter) / target; it is capped by max_increment and by what
+    is left to max_multiplier; the loop runs while multiplier < max_multiplier and increment >
+    min_increment.  The DOFs stay on the device between load steps; a diverged step is undone
+    by a device copy.  Returns (dofs, multiplier, increment, load_step, settings, res_norm) like
+    the reference's carry (its load_step is never incremented and stays 0)."""
+    import torch
+    if implicit_diff_mode is not None:
+        raise NotImplementedError("implicit differentiation of load stepping is not part of the 'hip' backend")
+    _check_backend(static_settings)
+    if static_settings["solver type"] not in ("newton", "damped newton"):
+        raise NotImplementedError("adaptive load stepping works with solver types 'newton' and 'damped newton'")
+    verbose = _opt(static_settings, "verbose", 0)
+    damping = 1.0 if static_settings["solver type"] == "newton" else float(kwargs.get("damping_coefficient", 0.8))
+    maxiter = int(kwargs.get("maxiter", 30))
+    prob = get_problem(dofs, settings, static_settings)
+    opts = newton_options(static_settings, newton_tol, maxiter, damping)
+    u = prob.to_device(prob.flat(dofs))
+    u_prev = torch.empty_like(u)
+    multiplier, increment, res = 0.0, float(init_increment), 0.0
+    steps = []
+    while True:
+        if multiplier < max_multiplier - min_increment and increment < min_increment:
+            print("Adaptive load stepping could not converge; increment size below min_increment!")
+        if not (multiplier < max_multiplier and increment > min_increment):
+            break
+        multiplier += increment
+        if verbose > -1:
+            print(f"Multiplier: {multiplier}")
+        settings = multiplier_settings(settings, multiplier)
+        prob.update(settings)
+        dv = prob.to_device(np.ascontiguousarray(utility.dict_flatten(settings["dirichlet conditions"]),
+                                                 dtype=np.float64))
+        u_prev.copy_(u)
+        st = N.NewtonStats()
+        N.call("apx_newton", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(dv.data_ptr()), C.byref(opts),
+               C.byref(st), current_stream())
+        needed, res, diverged = st.iterations, st.residual_norm, bool(st.diverged)
+        steps.append((multiplier, needed, diverged, st.krylov_iterations))
+        if diverged:
+            multiplier -= increment
+            increment *= 0.5
+            u.copy_(u_prev)
+        else:
+            increment *= 1.0 + 0.5 * (target_num_newton_iter - needed) / target_num_newton_iter
+        increment = min(increment, max_increment)
+        if multiplier + increment > max_multiplier:
+            increment = max_multiplier - multiplier
+    sol = utility.reshape_as(u.cpu().numpy(), dofs)
+    adaptive_load_stepping.last_steps = steps  # (multiplier, Newton iterations, diverged, Krylov iterations)
+    return sol, multiplier, increment, 0.0, settings, res
+
+
 def solve_newton(dofs, settings, static_settings, newton_tol=1e-8, maxiter=30, **kwargs):
     """solver.py:740-768 (damping 1.0)."""
     return solve_damped_newton(dofs, settings, static_settings, newton_tol, 1.0, maxiter)

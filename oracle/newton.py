@@ -131,3 +131,37 @@ def newton(u0, residual_fn, tangent_fn, dirichlet_mask, dirichlet_values, lin_so
         res_old = res
         not_stop = not_stop and nxt
     return dofs, (itt, res, div)
+
+
+def adaptive_load_stepping(u0, solve_step, max_multiplier=1.0, min_increment=0.01, max_increment=1.0,
+                           init_increment=0.2, target_num_newton_iter=7, log=None):
+    """solver.adaptive_load_stepping without implicit differentiation (solver.py:296-379, the
+    while_loop branch :455-457).  solve_step(u, multiplier) -> (u_new, (n_iter, res, diverged))
+    is one Newton solve at that load multiplier from u.
+
+    Loop while multiplier < max_multiplier and increment > min_increment:
+      multiplier += increment; solve;
+      diverged: multiplier -= increment, increment *= 0.5, keep the previous u;
+      else: increment *= 1 + 0.5 (target - n_iter) / target;
+      increment = min(increment, max_increment); if multiplier + increment > max_multiplier:
+      increment = max_multiplier - multiplier.
+    Returns (u, multiplier, increment, steps) with steps = [(multiplier, n_iter, diverged)] --
+    the reference returns its carry (dofs, multiplier, increment, load_step, settings, res)."""
+    u, multiplier, increment = np.array(u0, dtype=np.float64), 0.0, float(init_increment)
+    steps = []
+    while multiplier < max_multiplier and increment > min_increment:
+        multiplier += increment
+        u_new, (n_iter, res, diverged) = solve_step(u, multiplier)
+        steps.append((multiplier, int(n_iter), bool(diverged)))
+        if log:
+            log(f"multiplier {multiplier:.6f}: newton {n_iter} res {res:.3e} diverged {diverged}")
+        if diverged:
+            multiplier -= increment
+            increment *= 0.5
+        else:
+            u = u_new
+            increment *= 1.0 + 0.5 * (target_num_newton_iter - n_iter) / target_num_newton_iter
+        increment = min(increment, max_increment)
+        if multiplier + increment > max_multiplier:
+            increment = max_multiplier - multiplier
+    return u, multiplier, increment, steps

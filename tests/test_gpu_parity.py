@@ -186,3 +186,34 @@ def test_tapered_panel_traction_matches_oracle(torch_gpu):
     ref, rinfo = P.oracle_newton(o, tol=1e-9)
     assert infos[2] is False and rinfo[2] is False
     assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_adaptive_load_stepping_matches_oracle(torch_gpu):
+    """solver.adaptive_load_stepping on the device against the oracle's restatement: a Neo-Hooke
+    cube pulled to u_x = 2 with a first increment of 1 (which diverges) -- same multiplier
+    sequence, same Newton counts, same final solution."""
+    from autopdex_amd import solver
+    from oracle import newton as ON
+    n, total = 3, 2.0
+    dofs, settings, static, o = P.neo_hooke_cube(n, disp=total)
+    full = settings["dirichlet conditions"].copy()
+    static = dict(static, **{"linear tolerance": 1e-12, "verbose": -1})
+
+    def scale(s, m):
+        s["dirichlet conditions"] = full * m
+        return s
+
+    sol, m, inc, _, _, res = solver.adaptive_load_stepping(dofs, settings, static, multiplier_settings=scale,
+                                                           init_increment=1.0, newton_tol=1e-10)
+    dev_steps = solver.adaptive_load_stepping.last_steps
+
+    def oracle_step(u, mult):
+        oo = dict(o, values=o["values"] * mult)
+        return P.oracle_newton(oo, tol=1e-10, u0=u)
+
+    u_ref, m_ref, _, ref_steps = ON.adaptive_load_stepping(np.zeros(o["mask"].size), oracle_step, init_increment=1.0)
+    print(f"[case] device steps {[(round(a, 4), b, c) for a, b, c, _ in dev_steps]}", flush=True)
+    assert any(s[2] for s in ref_steps), "the first increment should diverge"
+    assert [(round(a, 12), b, c) for a, b, c, _ in dev_steps] == [(round(a, 12), b, c) for a, b, c in ref_steps]
+    assert m == m_ref == 1.0
+    assert np.linalg.norm(sol.ravel() - u_ref) / np.linalg.norm(u_ref) < 1e-8

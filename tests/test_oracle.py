@@ -274,3 +274,27 @@ def test_newton_divergence_and_maxiter_semantics():
     sol, info = ON.newton(np.zeros(mask.size), nan_res, lambda u: OA2.reduced_tangent(sets, coords, u, 3, ~mask),
                           mask, o["values"], lambda A, b: ON.linear_solve(A, np.nan_to_num(b))[0])
     assert info[2] is True and info[0] == 1
+
+
+def test_adaptive_load_stepping_increment_control():
+    """solver.py:296-379 increment control on a synthetic step function that diverges for
+    increments above 0.3 and otherwise needs 4 Newton iterations."""
+    state = {"last": 0.0}
+
+    def step(u, m):
+        inc = m - state["last"]
+        if inc > 0.3 + 1e-12:
+            return u, (31, np.inf, True)
+        state["last"] = m
+        return u + inc, (4, 1e-12, False)
+
+    u, m, inc, steps = ON.adaptive_load_stepping(np.zeros(1), step, init_increment=0.5)
+    # hand-checked prefix: 0.5 diverges -> back to 0, increment 0.25; 0.25 converges in 4
+    # iterations -> increment 0.25 (1 + 0.5 * 3 / 7); 0.5536 is 0.3036 past 0.25 and diverges
+    assert steps[0] == (0.5, 31, True)
+    assert steps[1] == (0.25, 4, False)
+    assert abs(steps[2][0] - (0.25 + 0.25 * (1 + 0.5 * 3 / 7))) < 1e-15 and steps[2][2]
+    assert m == 1.0 and abs(u[0] - 1.0) < 1e-12
+    accepted = [s[0] for s in steps if not s[2]]
+    assert all(b - a <= 0.3 + 1e-12 for a, b in zip([0.0] + accepted, accepted))
+    assert accepted == sorted(accepted)
