@@ -401,7 +401,14 @@ int apx_unet_apply(apx_unet* net, apx_plan* plan, const double* r_dev, const dou
     APX_REQUIRE(N.batch == 1, APX_ERR_ARG, "apply needs a batch-1 network");
     net_input_from_reduced_img(P, N, r_dev, 1, g_nodal_dev, nullptr, 0, S(stream));
     N.forward(S(stream));
-    net_output_to_reduced_img(P, N, z_dev, 1, 0, S(stream));
+    net_output_to_reduced_img(P, N, r_dev, z_dev, 1, 0, S(stream));
+  });
+}
+
+int apx_unet_set_jacobi_skip(apx_unet* net, int enable) {
+  return guarded([&] {
+    APX_REQUIRE(net, APX_ERR_ARG, "null network");
+    net->jacobi_skip = enable ? 1 : 0;
   });
 }
 

@@ -81,13 +81,14 @@ __global__ __launch_bounds__(kFinBS) void k_fcg_init_fin(KState* S, const double
 }
 
 // z = scale * dsq * y (grid -> free DOFs) and partials of z . q_prev
-__global__ void k_fcg_z(int64_t n, const int32_t* red2full, const float* y, const double* dsq, const float* scale,
-                        const double* q, double* z, KState* S, double* partials) {
+__global__ void k_fcg_z(int64_t n, const int32_t* red2full, const float* y, const double* dsq, const double* r_skip,
+                        const float* scale, const double* q, double* z, KState* S, double* partials) {
   if (S->done) return;
   const float s = scale[0];
   double v[1] = {0.0};
   for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    const double zi = (double)(s * y[red2full[i]]) * (dsq ? dsq[i] : 1.0);
+    double zi = (double)(s * y[red2full[i]]) * (dsq ? dsq[i] : 1.0);
+    if (r_skip) zi += r_skip[i] * dsq[i] * dsq[i];  // Jacobi part D^-1 r
     z[i] = zi;
     v[0] += zi * q[i];
   }
@@ -217,8 +218,8 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
     EventPair e0 = timing_begin(P, st);
     net_input_from_reduced_img(P, N, r, 1, g_nodal, &S->rr, 0, st);
     N.forward(st);
-    hipLaunchKernelGGL(k_fcg_z, dim3(gv), dim3(kBS), 0, st, n, P.red2full.p, N.y.p, P.net_dsq.p, N.scale.p, q, z, S,
-                       P.partials.p);
+    hipLaunchKernelGGL(k_fcg_z, dim3(gv), dim3(kBS), 0, st, n, P.red2full.p, N.y.p, P.net_dsq.p,
+                       N.jacobi_skip ? r : nullptr, N.scale.p, q, z, S, P.partials.p);
     timing_end(P, e0, st);
     hipLaunchKernelGGL(k_fcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv);
     EventPair e1 = timing_begin(P, st);

@@ -625,12 +625,14 @@ __global__ void k_to_grid(int64_t n_nodes, int d, int cin, const uint8_t* dmask,
 
 template <class T>
 __global__ void k_from_grid(int64_t n_free, int d, const int32_t* red2full, const float* y, const double* dsq,
-                            const float* scale, int img, int64_t n_nodes, T* z) {
+                            const double* r_skip, const float* scale, int img, int64_t n_nodes, T* z) {
   const float s = scale[img];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_free; i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t gd = red2full[i];
     const double v = (double)(s * y[(int64_t)img * n_nodes * d + gd]);
-    z[i] = (T)(dsq ? v * dsq[i] : v);
+    double zi = dsq ? v * dsq[i] : v;
+    if (r_skip) zi += r_skip[i] * dsq[i] * dsq[i];  // Jacobi part D^-1 r
+    z[i] = (T)zi;
   }
 }
 
@@ -714,17 +716,22 @@ void net_target_from_reduced_img(Plan& P, UNet& N, const double* e, float* targe
   APX_LAUNCH_CHECK();
 }
 
-void net_output_to_reduced_img(Plan& P, UNet& N, void* z, int z_f64, int img, hipStream_t st) {
+void net_output_to_reduced_img(Plan& P, UNet& N, const double* r, void* z, int z_f64, int img, hipStream_t st) {
   APX_REQUIRE(N.c_out == P.d, APX_ERR_ARG, "network output channels do not match the DOFs");
   int g;
   grid1d(P.n_free, g);
   const double* dsq = P.net_dsq_set ? P.net_dsq.p : nullptr;
+  const double* r_skip = nullptr;
+  if (N.jacobi_skip) {
+    APX_REQUIRE(dsq && r, APX_ERR_STATE, "the Jacobi part of the network needs the matrix diagonal and the residual");
+    r_skip = r;
+  }
   if (z_f64)
     hipLaunchKernelGGL(k_from_grid<double>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, dsq,
-                       N.scale.p, img, P.n_nodes, (double*)z);
+                       r_skip, N.scale.p, img, P.n_nodes, (double*)z);
   else
     hipLaunchKernelGGL(k_from_grid<float>, dim3(g), dim3(256), 0, st, P.n_free, P.d, P.red2full.p, N.y.p, dsq,
-                       N.scale.p, img, P.n_nodes, (float*)z);
+                       r_skip, N.scale.p, img, P.n_nodes, (float*)z);
   APX_LAUNCH_CHECK();
 }
 
@@ -816,7 +823,7 @@ double unet_residual_loss_backward(Plan& P, UNet& N, const double* vals_sell, co
   APX_HIP(hipMemcpy(hscale.data(), N.scale.p, N.batch * sizeof(float), hipMemcpyDeviceToHost));
   double loss = 0.0;
   for (int b = 0; b < N.batch; ++b) {
-    net_output_to_reduced_img(P, N, z, 1, b, st);
+    net_output_to_reduced_img(P, N, r[b], z, 1, b, st);
     spmv(P, vals_sell, z, w, APX_FP64, st);
     hipLaunchKernelGGL(k_res_terms, dim3(gr), dim3(256), 0, st, n, r[b], w, z, e ? e[b] : nullptr, res,
                        P.tpart.p);

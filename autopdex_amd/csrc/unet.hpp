@@ -44,6 +44,8 @@ struct UNet {
   // optional device stop flag (a Krylov solver's done flag): forward-pass kernels launched
   // after it is set return at once
   const int* stop = nullptr;
+  // 1: the network is a correction to Jacobi, z = D^-1 r + D^-1/2 s net(D^-1/2 r / s)
+  int jacobi_skip = 0;
 
   void build(int levels, const int* channels, int c_in, int c_out, int relu, int batch, int H0, int W0);
   void prepare_weights(hipStream_t st);   // FP32 master -> bf16 forward / backward-data layouts
@@ -70,7 +72,9 @@ void net_set_diag_scaling(Plan& P, const void* vals_sell, int precision, hipStre
 // rr_src (device, optional) holds sum r^2 over the free DOFs; s = sqrt(rr / n_free).
 void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, const double* g_nodal,
                                 const double* rr_src, int img, hipStream_t st);
-void net_output_to_reduced_img(Plan& P, UNet& N, void* z, int z_f64, int img, hipStream_t st);
+// r: the residual the input came from (FP64, reduced; used by the Jacobi skip, may be null
+// when the network has none)
+void net_output_to_reduced_img(Plan& P, UNet& N, const double* r, void* z, int z_f64, int img, hipStream_t st);
 // target t = e / s of image img (grid layout, FP32) for training
 void net_target_from_reduced_img(Plan& P, UNet& N, const double* e, float* target, int img, hipStream_t st);
 

@@ -36,7 +36,7 @@ class UNetPreconditioner:
     """Network on an (height x width) node grid; levels with channels[l] in {16, 32, 64}."""
 
     def __init__(self, height, width, levels=4, channels=(16, 32, 64, 64), c_in=1, c_out=1, relu=False, batch=1,
-                 seed=0):
+                 seed=0, jacobi_skip=False):
         torch_cuda()
         self.height, self.width, self.levels = int(height), int(width), int(levels)
         self.channels = tuple(int(c) for c in channels)[:levels]
@@ -46,6 +46,8 @@ class UNetPreconditioner:
         _call("apx_unet_create", C.byref(h), self.levels, ch, self.c_in, self.c_out, int(self.relu), self.batch,
               self.height, self.width)
         self.handle = h
+        self.jacobi_skip = bool(jacobi_skip)
+        _call("apx_unet_set_jacobi_skip", h, int(self.jacobi_skip))
         n, nl = C.c_int64(), C.c_int()
         _call("apx_unet_num_params", h, C.byref(n), C.byref(nl))
         self.n_params, self.n_layers = n.value, nl.value
@@ -133,7 +135,7 @@ class UNetPreconditioner:
 
     def spec(self):
         return dict(height=self.height, width=self.width, levels=self.levels, channels=list(self.channels),
-                    c_in=self.c_in, c_out=self.c_out, relu=self.relu)
+                    c_in=self.c_in, c_out=self.c_out, relu=self.relu, jacobi_skip=self.jacobi_skip)
 
     def save(self, path):
         theta = self.get_params()
@@ -150,7 +152,7 @@ class UNetPreconditioner:
         if hashlib.sha256(theta.tobytes()).hexdigest() != str(d["sha256"]):
             raise ValueError(f"{path}: parameter hash mismatch")
         net = cls(height or spec["height"], width or spec["width"], spec["levels"], spec["channels"], spec["c_in"],
-                  spec["c_out"], spec["relu"], batch=batch)
+                  spec["c_out"], spec["relu"], batch=batch, jacobi_skip=spec.get("jacobi_skip", False))
         net.set_params(theta)
         return net
 
@@ -226,7 +228,7 @@ def default_levels(height, width, coarsest=3):
 
 def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=16, n_smooth=16,
         pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
-        log=None, time_budget=None):
+        log=None, time_budget=None, jacobi_skip=False):
     """Train a linear U-Net preconditioner for `problem` (d DOFs per node on a structured
     height x width node grid: d input and output channels) and attach it.
 
@@ -266,9 +268,10 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     if log:
         log(f"[fit] {len(pairs)} (r, A^-1 r) pairs from {len(rhs)} plain solves in {t_pairs:.1f} s; levels {L}, "
             f"channels {ch}")
-    net = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=batch, seed=seed + 1)
+    net = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=batch, seed=seed + 1,
+                             jacobi_skip=jacobi_skip)
     net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
-    inf = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=1)
+    inf = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=1, jacobi_skip=jacobi_skip)
     inf.attach(problem)
     t1 = time.perf_counter()
     best, history, done = None, [], 0
@@ -302,7 +305,7 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
 
 def fit_fields(problem, height, width, fields, set_index=0, levels=None, channels=None, n_noise=2, n_smooth=2,
                pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
-               validation_field=None, log=None):
+               validation_field=None, log=None, jacobi_skip=False):
     """Train one network for a family of coefficient fields of a variable-coefficient model
     (models.poisson_variable_coefficient_integrand): the nodal log-coefficient g is the
     network's extra input channel, so it can follow the field it is applied to.
@@ -340,9 +343,10 @@ def fit_fields(problem, height, width, fields, set_index=0, levels=None, channel
         log(f"[fit_fields] {sum(len(d[2]) for d in data)} pairs on {len(data)} fields in {t_pairs:.1f} s; "
             f"levels {L}, channels {ch}")
     d = problem.n_dofs // (height * width)
-    net = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=batch, seed=seed + 1)
+    net = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=batch, seed=seed + 1,
+                             jacobi_skip=jacobi_skip)
     net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
-    inf = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=1)
+    inf = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=1, jacobi_skip=jacobi_skip)
     val = None
     if validation_field is not None:
         gv, fv = validation_field

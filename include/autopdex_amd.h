@@ -219,9 +219,16 @@ int apx_unet_load_pair(apx_unet* net, apx_plan* plan, const double* r_dev, const
 int apx_unet_train_step(apx_unet* net, apx_plan* plan, const void* vals_dev, const double* const* r_list,
                         const double* const* g_list, const double* const* e_list, int loss_kind, double w_e,
                         double* loss, void* stream);
-/* z = s * net(r / s) on the plan's free DOFs (FP64 in and out).            */
+/* z = D^-1/2 s net(D^-1/2 r / s) on the plan's free DOFs (FP64 in and out),
+ * s = RMS of D^-1/2 r, D = |diag A| of the matrix of the plan's latest
+ * flexible-CG solve or training step (none before the first: D = I).       */
 int apx_unet_apply(apx_unet* net, apx_plan* plan, const double* r_dev, const double* g_nodal_dev, double* z_dev,
                    void* stream);
+/* Make the network a correction to Jacobi: z = D^-1 r + D^-1/2 s net(D^-1/2 r / s)
+ * (enable = 1) instead of z = D^-1/2 s net(D^-1/2 r / s) (0, the default).  Applies to
+ * inference, flexible CG and training alike.                                */
+int apx_unet_set_jacobi_skip(apx_unet* net, int enable);
+
 /* Use `net` as preconditioner of the plan (apx_newton with precond
  * APX_PRECOND_LEARNED): flexible CG, Jacobi-PCG fallback on stagnation.     */
 int apx_plan_attach_unet(apx_plan* plan, apx_unet* net, const double* g_nodal_dev);

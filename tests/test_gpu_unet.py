@@ -184,3 +184,23 @@ def test_coefficient_family_network_generalizes_to_held_out_fields(torch_gpu):
         import scipy.sparse.linalg as spla
         x_ref = spla.spsolve(A.tocsc(), b.cpu().numpy())
         assert np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) < 1e-8
+
+
+def test_zero_network_with_jacobi_skip_is_jacobi_pcg(torch_gpu):
+    """With the Jacobi skip and all-zero weights the preconditioner is exactly D^-1, and flexible
+    CG(1) with a fixed SPD preconditioner is CG: same iteration count as Jacobi-PCG (within 2)
+    and the same solution."""
+    torch = torch_gpu
+    from autopdex_amd import learned
+    n = 32
+    dofs, settings, static, o, prob = _poisson_problem(n)
+    vals = learned.tangent(prob)
+    net = learned.UNetPreconditioner(n + 1, n + 1, 3, (16, 32, 64), 1, 1, batch=1, jacobi_skip=True)
+    net.set_params(np.zeros(net.n_params, dtype=np.float32))
+    net.attach(prob)
+    b = torch.from_numpy(np.random.default_rng(5).standard_normal(prob.n_free)).cuda()
+    x_f, ks_f = learned.fcg(prob, vals, b, tol=1e-10, maxiter=2000)
+    x_p, ks_p = learned.pcg(prob, vals, b, tol=1e-10)
+    print(f"[jacobi skip] FCG {ks_f.iterations} it, PCG {ks_p.iterations} it", flush=True)
+    assert ks_f.converged and abs(ks_f.iterations - ks_p.iterations) <= 2
+    assert float(torch.linalg.norm(x_f - x_p) / torch.linalg.norm(x_p)) < 1e-8
