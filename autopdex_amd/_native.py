@@ -57,6 +57,8 @@ _SIGS = {
     "apx_assemble_residual": (_int, [_vp, _vp, _vp, _vp]),
     "apx_assemble_tangent": (_int, [_vp, _vp, _vp, _int, _vp]),
     "apx_assemble_diagonal": (_int, [_vp, _vp, _vp, _vp]),
+    "apx_tangent_coo_entries": (_int, [_vp, C.POINTER(_i64)]),
+    "apx_assemble_tangent_coo": (_int, [_vp, _vp, _vp, _int, _vp]),
     "apx_spmv": (_int, [_vp, _vp, _vp, _vp, _int, _vp]),
     "apx_pcg": (_int, [_vp, _vp, _vp, _vp, _int, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
     "apx_newton": (_int, [_vp, _vp, _vp, C.POINTER(NewtonOptions), C.POINTER(NewtonStats), _vp]),

@@ -13,7 +13,7 @@
 
 namespace apx {
 
-enum { MODE_RES = 0, MODE_TAN = 1, MODE_DIAG = 2 };
+enum { MODE_RES = 0, MODE_TAN = 1, MODE_DIAG = 2, MODE_COO = 3 };
 
 struct AsmArgs {
   const double* coords;  // SoA, dim x n_nodes
@@ -68,6 +68,13 @@ __global__ __launch_bounds__(BS) void k_team(AsmArgs A) {
     if constexpr (MODE == MODE_DIAG) {
 #pragma unroll
       for (int i = 0; i < D; ++i) A.r[(int64_t)node * D + i] += (double)out[i][t * D + i];
+    } else if constexpr (MODE == MODE_COO) {
+      // element-major, row-major block (the reference's BCOO order); every entry written once
+      T* data = (T*)A.vals + ((int64_t)e * ND + (int64_t)t * D) * ND;
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int B = 0; B < ND; ++B) data[i * ND + B] = out[i][B];
     } else {
       T* vals = (T*)A.vals;
       const int32_t* sl = A.slots + (e * ND + (int64_t)t * D) * ND;
@@ -114,6 +121,19 @@ static void run_set(Plan& P, ElemSet& S, const double* u, double* r, void* vals,
   if constexpr (M::NODAL) {
     APX_REQUIRE(S.params.nodal[0] && S.params.nodal[1], APX_ERR_STATE,
                 "variable-coefficient Poisson set needs nodal fields g and f bound");
+  }
+  if constexpr (MODE == MODE_COO) {
+    // element blocks go to disjoint slots: one launch over the original element order
+    if (S.conn_orig.n < (size_t)(S.n_elems * S.npe)) {
+      std::vector<int32_t> c32(S.conn_host.begin(), S.conn_host.end());
+      S.conn_orig.upload(c32.data(), c32.size());
+    }
+    A.conn = S.conn_orig.p;
+    A.e0 = 0;
+    A.e1 = S.n_elems;
+    hipLaunchKernelGGL((k_team<M, T, MODE, BS>), dim3((S.n_elems + EPB - 1) / EPB), dim3(BS), 0, st, A);
+    APX_LAUNCH_CHECK();
+    return;
   }
   for (size_t c = 0; c + 1 < S.colour_offsets.size(); ++c) {
     A.e0 = S.colour_offsets[c];
@@ -235,6 +255,26 @@ void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipSt
     fill<float>((float*)vals, P.sell_total, 0.0f, st);
     for (auto& S : P.sets)
       if (S.model != APX_MODEL_LINE_TRACTION) dispatch<float, MODE_TAN>(P, S, u, nullptr, vals, st);
+  }
+}
+
+int64_t tangent_coo_entries(const Plan& P) {
+  int64_t n = 0;
+  for (auto& S : P.sets) n += S.n_elems * ndofs_elem(P, S) * ndofs_elem(P, S);
+  return n;
+}
+
+// The full tangent as the reference's BCOO with duplicates: per set, element-major blocks
+// (rows repeat(gdofs), columns tile(gdofs)); sets concatenated in order.
+void assemble_tangent_coo(Plan& P, const double* u, void* data, int precision, hipStream_t st) {
+  int64_t off = 0;
+  for (auto& S : P.sets) {
+    const int64_t nd = ndofs_elem(P, S);
+    if (precision == APX_FP64)
+      dispatch<double, MODE_COO>(P, S, u, nullptr, (double*)data + off, st);
+    else
+      dispatch<float, MODE_COO>(P, S, u, nullptr, (float*)data + off, st);
+    off += S.n_elems * nd * nd;
   }
 }
 

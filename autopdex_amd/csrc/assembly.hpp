@@ -10,6 +10,9 @@ void fill(T* p, int64_t n, T v, hipStream_t s);
 void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st);
 void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st);
 void assemble_diagonal(Plan& P, const double* u, double* diag, hipStream_t st);
+// full tangent as BCOO with duplicates (data only; the reference's element-major order)
+int64_t tangent_coo_entries(const Plan& P);
+void assemble_tangent_coo(Plan& P, const double* u, void* data, int precision, hipStream_t st);
 int64_t residual_bytes(const Plan& P);
 int64_t tangent_bytes(const Plan& P, int precision);
 // one set's residual through the colour kernels (r += ...)

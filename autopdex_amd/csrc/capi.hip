@@ -179,6 +179,22 @@ int apx_assemble_tangent(apx_plan* plan, const double* u_dev, void* vals_dev, in
   });
 }
 
+int apx_tangent_coo_entries(const apx_plan* plan, int64_t* count) {
+  return guarded([&] {
+    APX_REQUIRE(plan && count, APX_ERR_ARG, "null argument");
+    *count = tangent_coo_entries(*plan);
+  });
+}
+
+int apx_assemble_tangent_coo(apx_plan* plan, const double* u_dev, void* data_dev, int precision, void* stream) {
+  return guarded([&] {
+    Plan& P = ready(plan);
+    APX_REQUIRE(u_dev && data_dev, APX_ERR_ARG, "null device pointer");
+    APX_REQUIRE(precision == APX_FP64 || precision == APX_FP32, APX_ERR_ARG, "precision must be FP64 or FP32");
+    assemble_tangent_coo(P, u_dev, data_dev, precision, S(stream));
+  });
+}
+
 int apx_assemble_diagonal(apx_plan* plan, const double* u_dev, double* diag_dev, void* stream) {
   return guarded([&] {
     Plan& P = ready(plan);

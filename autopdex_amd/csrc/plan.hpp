@@ -24,6 +24,7 @@ struct ElemSet {
   // colour-permuted device data
   DevBuf<int32_t> conn;            // n_elems * npe, colour-major order
   DevBuf<int32_t> slots;           // n_elems * (npe*d)^2 reduced CSR slots, -1 = Dirichlet
+  DevBuf<int32_t> conn_orig;       // original element order (made on first BCOO export)
   std::vector<int64_t> colour_offsets;  // n_colours + 1
   // Structured Q1 grid (the connectivity of mesher.structured_mesh): element
   // e = i * gny + j has nodes b, b + 1, b + gny + 2, b + gny + 1 with

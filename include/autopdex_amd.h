@@ -113,6 +113,17 @@ int apx_assemble_tangent(apx_plan* plan, const double* u_dev, void* vals_dev, in
  * diag_dev (n_dofs, FP64, full length).                                     */
 int apx_assemble_diagonal(apx_plan* plan, const double* u_dev, double* diag_dev, void* stream);
 
+/* assembler.assemble_tangent as the reference returns it (assembler.py:682-777,
+ * user_potential_assemble_tangent :1115-1149, indices _get_indices :47-141):
+ * the full n_dofs x n_dofs tangent as BCOO with duplicates.  data_dev receives
+ * the values in the reference's order -- per set, element-major row-major
+ * blocks, rows repeat(gdofs, n), columns tile(gdofs, n) with gdofs = node*d +
+ * comp (P11); sets concatenated -- apx_tangent_coo_entries() values in all
+ * (precision = apx_precision).  The indices depend only on the connectivity
+ * and are built by the caller.                                              */
+int apx_tangent_coo_entries(const apx_plan* plan, int64_t* count);
+int apx_assemble_tangent_coo(apx_plan* plan, const double* u_dev, void* data_dev, int precision, void* stream);
+
 /* ---- linear algebra on the reduced system -------------------------------- */
 
 /* y = A x with A the reduced CSR of the plan (vals in `precision`).         */

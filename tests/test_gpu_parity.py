@@ -217,3 +217,30 @@ def test_adaptive_load_stepping_matches_oracle(torch_gpu):
     assert [(round(a, 12), b, c) for a, b, c, _ in dev_steps] == [(round(a, 12), b, c) for a, b, c in ref_steps]
     assert m == m_ref == 1.0
     assert np.linalg.norm(sol.ravel() - u_ref) / np.linalg.norm(u_ref) < 1e-8
+
+
+@pytest.mark.parametrize("name,builder,args", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_tangent_bcoo_export_matches_reference_order(torch_gpu, name, builder, args):
+    """assembler.assemble_tangent(form='bcoo'): the full matrix with duplicates, entry for entry in
+    the reference's order (per set element-major row-major blocks, sets concatenated)."""
+    from autopdex_amd import assembler
+    from oracle import assembly as OA
+    dofs, settings, static, o, prob = _setup(builder, *args)
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(prob.n_dofs) * (0.02 if o["d"] > 1 else 1.0)
+    M = assembler.assemble_tangent(utility_reshape(u, dofs), settings, static, form="bcoo")
+    rows, cols, data = [], [], []
+    for model, conn, nodal in o["sets"]:
+        r_, c_, d_ = OA.tangent_coo(model, o["coords"], conn, u, o["d"], nodal)
+        rows.append(r_), cols.append(c_), data.append(d_)
+    rows, cols, data = np.concatenate(rows), np.concatenate(cols), np.concatenate(data)
+    assert M.shape == (prob.n_dofs, prob.n_dofs)
+    assert np.array_equal(M.row, rows) and np.array_equal(M.col, cols)
+    assert _rel(M.data, data) < 1e-12
+    M32 = assembler.assemble_tangent(utility_reshape(u, dofs), settings, static, precision="fp32", form="bcoo")
+    assert _rel(M32.data.astype(np.float64), data) < 1e-5
+
+
+def utility_reshape(u, dofs):
+    from autopdex_amd import utility
+    return utility.reshape_as(u, dofs)
