@@ -204,3 +204,22 @@ def test_zero_network_with_jacobi_skip_is_jacobi_pcg(torch_gpu):
     print(f"[jacobi skip] FCG {ks_f.iterations} it, PCG {ks_p.iterations} it", flush=True)
     assert ks_f.converged and abs(ks_f.iterations - ks_p.iterations) <= 2
     assert float(torch.linalg.norm(x_f - x_p) / torch.linalg.norm(x_p)) < 1e-8
+
+
+def test_learned_mixed_newton_reaches_fp64_accuracy(torch_gpu):
+    """The bench's headline method through solver.solver: FP32 tangent, FP32-matrix flexible CG
+    with a network from learned.fit, inner solves re-anchored by the FP64 Newton residual --
+    converges to the FP64 tolerance and to the oracle's solution (1e-8)."""
+    from autopdex_amd import learned, solver
+    n = 64
+    dofs, settings, static, o, prob = _poisson_problem(n)
+    net, info = learned.fit(prob, n + 1, n + 1, n_noise=4, n_smooth=4, steps=400, validate_every=200)
+    s = dict(static, **{"type of preconditioner": "learned", "hip precision": "mixed", "reliable delta": 1e-2,
+                        "linear tolerance": 1e-10, "linear forcing": True, "newton tolerance mode": "relative",
+                        "linear maxiter": 2000})
+    sol, infos = solver.solver(dofs, settings, s, newton_tol=1e-10, maxiter=50)
+    ref, _ = P.oracle_newton(o, tol=1e-12)
+    print(f"[learned mixed] newton {infos[0]} krylov {infos.details['krylov_iterations']} "
+          f"fallbacks {infos.details['fallbacks']}", flush=True)
+    assert infos[2] is False and infos[1] <= 1e-10 * infos.details["initial_residual"]
+    assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
