@@ -34,15 +34,46 @@ int kblocks(int ci) { return ci >= 16 ? 9 * (ci / 16) : (9 * ci + 15) / 16; }
 // 16-channel blocks (more blocks, 1/NT of the weights and MFMAs each).
 constexpr int64_t kSplitTiles = 128;
 
+// Blocks of one launch: at most what the device holds at once (the occupancy of
+// the kernel: LDS and registers), so that no block starts after the first round
+// has finished (a second partial round would run at low occupancy).
+template <class K>
+int64_t resident_blocks(K kernel) {
+  int per_cu = 0, dev = 0, cus = 0;
+  APX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kConvThreads, 0));
+  APX_HIP(hipGetDevice(&dev));
+  APX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return (int64_t)std::max(1, cus) * std::max(1, per_cu);
+}
+
 template <int CI, int CO, bool F32>
 void launch_conv_t(const ConvArgs& a, hipStream_t st) {
   constexpr int NT = round16(CO) / 16;
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, 2048));
-  if (NT > 1 && tiles < kSplitTiles)
+  if (NT > 1 && tiles < kSplitTiles) {
+    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, 1>);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap / NT));
     hipLaunchKernelGGL((k_conv3x3<CI, CO, F32, 1>), dim3(g, NT), dim3(kConvThreads), 0, st, a);
-  else
+  } else {
+    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT>);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
     hipLaunchKernelGGL((k_conv3x3<CI, CO, F32, NT>), dim3(g), dim3(kConvThreads), 0, st, a);
+  }
+}
+
+template <int CI, int CO, int TR>
+void launch_conv_ws_t(const ConvArgs& a, hipStream_t st) {
+  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + TR - 1) / TR);
+  static const int64_t cap = resident_blocks(k_conv3x3_ws<CI, CO, TR>);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+  hipLaunchKernelGGL((k_conv3x3_ws<CI, CO, TR>), dim3(g), dim3(kConvThreads), 0, st, a);
+}
+
+// Four-row tiles: measured faster than eight-row tiles (which hold twice the accumulators and
+// run at lower occupancy) on the 2049 x 2049 level-0 layers.
+template <int CI, int CO>
+void launch_conv_ws(const ConvArgs& a, hipStream_t st) {
+  launch_conv_ws_t<CI, CO, 4>(a, st);
 }
 
 template <int CI, int CO>
@@ -66,6 +97,10 @@ void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
     if (ci == 16 && co == 2) return launch_conv_t<16, 2, true>(a, st);
     throw Error(APX_ERR_UNSUPPORTED, "FP32-output convolution only for the 16 -> 1|2 output layer");
   }
+  if (ci == 16 && co == 16) return launch_conv_ws<16, 16>(a, st);
+  if (ci == 16 && co == 32) return launch_conv_ws<16, 32>(a, st);
+  if (ci == 32 && co == 16) return launch_conv_ws<32, 16>(a, st);
+  if (ci == 32 && co == 32) return launch_conv_ws<32, 32>(a, st);
 #define APX_X(A, B) \
   if (ci == A && co == B) return launch_conv_t<A, B, false>(a, st);
   APX_CONV_CASES(APX_X)

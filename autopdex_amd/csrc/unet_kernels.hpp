@@ -97,21 +97,15 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   uint16_t* sx = sbuf;
   const int co0 = blockIdx.y * 16 * NB;  // first output channel of this block
-  // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; all
-  // of a thread's loads are issued before its stores so that they overlap
-  {
-    constexpr int NW = 16 * NB * (WK / 8), RW = (NW + kConvThreads - 1) / kConvThreads;
-    u32x4 q[RW];
+  // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; the
+  // weight loads and the first tile's loads are all issued before the weights are stored, so
+  // that the two round trips overlap (it matters on small grids, where a launch has one tile)
+  constexpr int NW = 16 * NB * (WK / 8), RW = (NW + kConvThreads - 1) / kConvThreads;
+  u32x4 qw[RW];
 #pragma unroll
-    for (int r = 0; r < RW; ++r) {
-      const int i = threadIdx.x + r * kConvThreads;
-      if (i < NW) q[r] = *(const u32x4*)(a.w + (int64_t)(co0 + i / (WK / 8)) * WK + (i % (WK / 8)) * 8);
-    }
-#pragma unroll
-    for (int r = 0; r < RW; ++r) {
-      const int i = threadIdx.x + r * kConvThreads;
-      if (i < NW) *(u32x4*)(sw + (i / (WK / 8)) * SWK + (i % (WK / 8)) * 8) = q[r];
-    }
+  for (int r = 0; r < RW; ++r) {
+    const int i = threadIdx.x + r * kConvThreads;
+    if (i < NW) qw[r] = *(const u32x4*)(a.w + (int64_t)(co0 + i / (WK / 8)) * WK + (i % (WK / 8)) * 8);
   }
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + kTileH - 1) / kTileH;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
@@ -163,11 +157,16 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   };
   const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
   if (tile0 < ntiles) load_tile(tile0);
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int i = threadIdx.x + r * kConvThreads;
+    if (i < NW) *(u32x4*)(sw + (i / (WK / 8)) * SWK + (i % (WK / 8)) * 8) = qw[r];
+  }
   for (int64_t tile = tile0; tile < ntiles; tile += gridDim.x) {
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
     const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;
-    __syncthreads();  // previous tile's output copy done (also orders the weight load)
+    __syncthreads();  // previous tile's output copy done (also orders the weight stores)
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
       const int i = threadIdx.x + r * kConvThreads;
@@ -250,6 +249,154 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
         uint16_t* dst = (uint16_t*)a.y + row0;
         for (int i = lane; i < npx * C8; i += 64)
           *(u32x4*)(dst + (i / C8) * CO + (i % C8) * 8) = *(const u32x4*)(src + 8 * i);
+      }
+    }
+  }
+}
+
+// ---- weight-stationary convolution for 16- and 32-channel layers ---------------
+// The same operator as k_conv3x3 (bf16 output) for CI, CO in {16, 32}, arranged
+// for less on-chip traffic per MFMA:
+//  - the weights are the A operand (M = output channels) and stay in registers
+//    for the whole launch: lane (l%16, l/16) holds W[16n + l%16][t CI + 16c + 4(l/16) + j];
+//  - pixels are the N dimension: a B fragment is 16 consecutive pixels of one
+//    staged row, and the D fragment (pixel l%16, channels 16n + 4(l/16) + j) is
+//    stored straight to global memory, 16 pixels x 32 B contiguous per store;
+//  - each wavefront owns a 16-pixel column of the TR x 64 tile and all TR output
+//    rows, so one fragment of staged row yi feeds output rows yi-2 .. yi;
+//  - the staged tile is planar in channel quads: plane q holds channels 4q..4q+3
+//    of every staged pixel (8 B each), so the 16 lanes of one quad read 128
+//    contiguous bytes, and consecutive planes start 128 B apart modulo 256 B.
+// Two barriers per tile; the next tile's global loads are in flight in registers
+// during the current tile's MFMAs.
+template <int TR>
+__host__ __device__ constexpr int ws_plane() {
+  // (TR + 2) x (kTileW + 2) staged pixels, rounded up to 16 (mod 32)
+  return (((TR + 2) * (kTileW + 2) + 15) / 32) * 32 + 16;
+}
+
+template <int CI, int CO, int TR>
+__global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
+  static_assert((CI == 16 || CI == 32) && (CO == 16 || CO == 32), "16- or 32-channel layers");
+  constexpr int NC = CI / 16, NO = CO / 16;  // K blocks per tap, output channel blocks
+  constexpr int TWP = kTileW + 2;            // staged row length (pixels)
+  constexpr int NP = (TR + 2) * TWP;         // staged pixels
+  constexpr int PP = ws_plane<TR>();         // pixels per plane
+  static_assert(PP >= NP, "plane holds the staged tile");
+  constexpr int CH8 = CI / 8;                // 16-byte chunks per pixel
+  constexpr int NL = (NP + 15) / 16 * 16 * CH8;
+  constexpr int RC = (NL + kConvThreads - 1) / kConvThreads;
+  constexpr int WK = 9 * CI;                 // weight row length
+  __shared__ __attribute__((aligned(16))) uint16_t sx[(CI / 4) * PP * 4];
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  bf16x4 wa[9][NC][NO];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int n = 0; n < NO; ++n)
+        wa[t][c][n] = *(const bf16x4*)(a.w + (int64_t)(16 * n + lr) * WK + t * CI + 16 * c + 4 * lq);
+  float bb[NO][4];
+#pragma unroll
+  for (int n = 0; n < NO; ++n)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bb[n][j] = a.bias ? a.bias[16 * n + 4 * lq + j] : 0.f;
+
+  const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + TR - 1) / TR;
+  const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
+  // load slot i: 16-lane group g = i / 16 takes chunk g % CH8 of pixels 16 (g / CH8) + i % 16
+  u32x4 qs[RC];
+  auto load_tile = [&](int64_t tile) {
+    const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
+    const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
+    const int ty0 = (trem / tiles_x) * TR, tx0 = (trem % tiles_x) * kTileW;
+    const int64_t img_off = (int64_t)img * a.H * a.W * CI;
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      const int g = i >> 4, h = g % CH8, pix = (g / CH8) * 16 + (i & 15);
+      const int yy = ty0 - 1 + pix / TWP, xx = tx0 - 1 + pix % TWP;
+      u32x4 q = u32x4{0u, 0u, 0u, 0u};
+      if (i < NL && pix < NP && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+        const int64_t o = img_off + ((int64_t)yy * a.W + xx) * CI + 8 * h;
+        q = *(const u32x4*)(a.x + o);
+        if (a.xmask) {
+          const u32x4 m = *(const u32x4*)(a.xmask + o);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            uint32_t keep = 0u;
+            if (bf2f((uint16_t)(m[e] & 0xffffu)) > 0.f) keep |= 0xffffu;
+            if (bf2f((uint16_t)(m[e] >> 16)) > 0.f) keep |= 0xffff0000u;
+            q[e] &= keep;
+          }
+        }
+      }
+      qs[r] = q;
+    }
+  };
+  const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
+  if (tile0 < ntiles) load_tile(tile0);
+  for (int64_t tile = tile0; tile < ntiles; tile += gridDim.x) {
+    const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
+    const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
+    const int ty0 = (trem / tiles_x) * TR, tx0 = (trem % tiles_x) * kTileW;
+    __syncthreads();  // every wavefront is done reading the previous tile
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      const int g = i >> 4, h = g % CH8, pix = (g / CH8) * 16 + (i & 15);
+      if (i < NL && pix < NP) {
+        *(uint2*)(sx + ((2 * h) * PP + pix) * 4) = uint2{qs[r][0], qs[r][1]};
+        *(uint2*)(sx + ((2 * h + 1) * PP + pix) * 4) = uint2{qs[r][2], qs[r][3]};
+      }
+    }
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);  // in flight during the MFMAs
+    f32x4 acc[TR][NO];
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+      for (int n = 0; n < NO; ++n) acc[r][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int yi = 0; yi < TR + 2; ++yi)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const bf16x4 f = *(const bf16x4*)(sx + ((4 * c + lq) * PP + yi * TWP + 16 * wave + lr + dx) * 4);
+#pragma unroll
+          for (int dy = 0; dy < 3; ++dy) {
+            const int r = yi - dy;  // output row fed by staged row yi through tap row dy
+            if (r >= 0 && r < TR) {
+#pragma unroll
+              for (int n = 0; n < NO; ++n)
+                acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[dy * 3 + dx][c][n], f, acc[r][n], 0, 0, 0);
+            }
+          }
+        }
+    const int xx = tx0 + 16 * wave + lr;
+    if (xx < a.W) {
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const int yy = ty0 + r;
+        if (yy < a.H) {
+          uint16_t* dst = (uint16_t*)a.y + (((int64_t)img * a.H + yy) * a.W + xx) * CO + 4 * lq;
+#pragma unroll
+          for (int n = 0; n < NO; ++n) {
+            uint16_t o[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              float v = acc[r][n][j] + bb[n][j];
+              if (a.relu) v = fmaxf(v, 0.f);
+              o[j] = f2bf(v);
+            }
+            *(uint2*)(dst + 16 * n) =
+                uint2{(uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16)};
+          }
+        }
       }
     }
   }
