@@ -53,10 +53,13 @@ class DeviceProblem:
         self.d = 1 if u.ndim == 1 else u.shape[-1]
         if u.shape[0] != self.n_nodes:
             raise ValueError("dofs and node coordinates disagree on the number of nodes")
-        modes = static_settings["assembling mode"]
-        if any(m != "user potential" for m in modes):
-            raise NotImplementedError("the 'hip' backend implements the 'user potential' assembling mode")
+        modes = list(static_settings["assembling mode"])
+        if any(m not in ("user potential", "user element") for m in modes):
+            raise NotImplementedError("the 'hip' backend implements the 'user potential' and 'user element' "
+                                      "assembling modes")
         self.models = list(static_settings["model"])
+        if len(modes) == 1 and len(self.models) > 1:
+            modes = modes * len(self.models)
         mask = np.ascontiguousarray(utility.dict_flatten(settings["dirichlet dofs"]).astype(np.uint8))
         if mask.size != self.n_nodes * self.d:
             raise ValueError("'dirichlet dofs' must have the structure of dofs")
@@ -66,17 +69,25 @@ class DeviceProblem:
         self.plan = plan
         self._keep = [coords, mask]
         self.nodal = {}
-        for i, (model, conn_s) in enumerate(zip(self.models, settings["connectivity"])):
-            if not isinstance(model, (M.UserPotential, M.LineTractionPotential)):
+        self.samples = []
+        for i, (model, conn_s, mode) in enumerate(zip(self.models, settings["connectivity"], modes)):
+            if mode == "user potential" and not isinstance(model, (M.UserPotential, M.LineTractionPotential)):
                 raise NotImplementedError(
                     "the 'hip' backend needs autopdex_amd.models descriptors "
                     "(mixed_reference_domain_potential of an autopdex_amd integrand, or line_traction_potential)")
+            if mode == "user element" and not isinstance(model, (M.UserElement, M.UserSurfaceElement)):
+                raise NotImplementedError(
+                    "'user element' sets need isoparametric_domain_element_galerkin or "
+                    "isoparametric_surface_element_galerkin of autopdex_amd.models")
             if model.dofs_per_node(self.dim) != self.d:
                 raise ValueError(f"model of set {i} expects {model.dofs_per_node(self.dim)} DOFs per node")
             conn = np.ascontiguousarray(_pick(conn_s, self.key), dtype=np.int64)
-            params = np.asarray(model.params(settings, self.dim), dtype=np.float64)
+            kernel = model.kernel_for(conn.shape[1], self.dim) if hasattr(model, "kernel_for") else model.kernel
+            # user functions of x are evaluated at the set's distinct nodes (at most 64 of them)
+            self.samples.append(coords[np.unique(conn)[:64]])
+            params = self._params(i, model, settings)
             sid = C.c_int()
-            N.call("apx_plan_add_set", plan, conn.ctypes.data, conn.shape[0], conn.shape[1], model.kernel,
+            N.call("apx_plan_add_set", plan, conn.ctypes.data, conn.shape[0], conn.shape[1], kernel,
                    params.ctypes.data if params.size else None, params.size, C.byref(sid))
             self._keep.append(conn)
         N.call("apx_plan_finalize", plan)
@@ -87,11 +98,27 @@ class DeviceProblem:
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.update(settings)
 
+    def raw_params(self, i, settings):
+        """Parameters of set i as the descriptor computes them from settings (complex when a
+        setting is complex: complex-step differentiation in autopdex_amd.implicit_diff)."""
+        model = self.models[i]
+        if getattr(model, "user_element", False):
+            return list(model.params(settings, self.dim, sample_x=self.samples[i]))
+        return list(model.params(settings, self.dim))
+
+    def set_params(self, i, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        if p.size:
+            N.call("apx_plan_set_params", self.plan, i, p.ctypes.data, p.size)
+
+    def _params(self, i, model, settings):
+        return np.asarray(self.raw_params(i, settings), dtype=np.float64)
+
     def update(self, settings):
         """Refresh per-call parameters (material, load multiplier) and nodal fields."""
         torch = torch_cuda()
         for i, model in enumerate(self.models):
-            params = np.asarray(model.params(settings, self.dim), dtype=np.float64)
+            params = self._params(i, model, settings)
             if params.size:
                 N.call("apx_plan_set_params", self.plan, i, params.ctypes.data, params.size)
             fields = model.nodal_fields(settings)

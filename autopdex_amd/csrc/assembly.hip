@@ -107,8 +107,9 @@ template void fill<float>(float*, int64_t, float, hipStream_t);
 
 template <class M, class T, int MODE>
 static void run_set(Plan& P, ElemSet& S, const double* u, double* r, void* vals, hipStream_t st) {
-  constexpr int BS = (M::NPE == 8) ? 64 : 128;
-  constexpr int EPB = BS / M::NPE;
+  // whole elements per block: every thread's element slot le = threadIdx.x / NPE is < EPB
+  constexpr int EPB = (M::NPE >= 8 ? 64 : 128) / M::NPE;
+  constexpr int BS = EPB * M::NPE;
   AsmArgs A;
   A.coords = P.coords.p;
   A.n_nodes = P.n_nodes;
@@ -167,9 +168,15 @@ static void dispatch(Plan& P, ElemSet& S, const double* u, double* r, void* vals
     case APX_MODEL_NEO_HOOKE * 100 + 83:
       APX_REQUIRE(P.d == 3, APX_ERR_ARG, "3D Neo-Hooke needs 3 DOFs per node");
       return run_set<NeoHooke<8, 3>, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_NEO_HOOKE * 100 + 92:
+      APX_REQUIRE(P.d == 2, APX_ERR_ARG, "plane-strain Neo-Hooke needs 2 DOFs per node");
+      return run_set<NeoHooke<9, 2>, T, MODE>(P, S, u, r, vals, st);
     case APX_MODEL_LINE_TRACTION * 100 + 22:
       APX_REQUIRE(P.d == 2, APX_ERR_ARG, "line traction needs 2 DOFs per node");
       return run_set<LineTraction, T, MODE>(P, S, u, r, vals, st);
+    case APX_MODEL_LINE_TRACTION * 100 + 32:
+      APX_REQUIRE(P.d == 2, APX_ERR_ARG, "line traction needs 2 DOFs per node");
+      return run_set<LineTraction3, T, MODE>(P, S, u, r, vals, st);
     default:
       throw Error(APX_ERR_UNSUPPORTED, "no kernel for model " + std::to_string(S.model) + " with " +
                                            std::to_string(S.npe) + "-node elements in " +

@@ -64,7 +64,8 @@ int apx_plan_add_set(apx_plan* plan, const int64_t* conn_host, int64_t n_elems, 
     APX_REQUIRE(plan && conn_host, APX_ERR_ARG, "null argument");
     APX_REQUIRE(!plan->finalized, APX_ERR_STATE, "plan already finalized");
     APX_REQUIRE(n_elems > 0, APX_ERR_ARG, "empty element set");
-    APX_REQUIRE(npe == 2 || npe == 4 || npe == 8, APX_ERR_UNSUPPORTED, "nodes_per_elem must be 2, 4 or 8");
+    APX_REQUIRE(npe == 2 || npe == 3 || npe == 4 || npe == 8 || npe == 9, APX_ERR_UNSUPPORTED,
+                "nodes_per_elem must be 2, 3, 4, 8 or 9");
     APX_REQUIRE(n_params >= 0 && n_params <= kMaxParams, APX_ERR_ARG, "too many parameters");
     APX_REQUIRE(model >= APX_MODEL_POISSON_SOURCE && model <= APX_MODEL_LINE_TRACTION, APX_ERR_UNSUPPORTED,
                 "unknown model id");

@@ -20,6 +20,24 @@ constexpr double kGaussA = 2.0 * 0.21132486540518713 - 1.0;
 constexpr double kGaussB = 2.0 * 0.7886751345948129 - 1.0;
 __device__ __forceinline__ double gauss_x(int i) { return i ? kGaussB : kGaussA; }
 
+// 3-point Gauss-Legendre rule on [-1, 1]: 2 p - 1 and 2 w of the order-4|5 rule
+// of seeder.gauss_legendre_1d (seeder.py:387-393), digits as the reference holds them.
+constexpr double kGauss3A = 2.0 * 0.1127016653792583 - 1.0;
+constexpr double kGauss3C = 2.0 * 0.8872983346207417 - 1.0;
+constexpr double kGauss3WE = 2.0 * 0.2777777777777777, kGauss3WM = 2.0 * 0.44444444444444453;
+__device__ __forceinline__ double gauss3_x(int i) { return i == 0 ? kGauss3A : (i == 1 ? 0.0 : kGauss3C); }
+__device__ __forceinline__ double gauss3_w(int i) { return i == 1 ? kGauss3WM : kGauss3WE; }
+
+// 1D quadratic Lagrange polynomials on the nodes -1, 0, 1 and their derivatives.
+__device__ __forceinline__ void lagrange3(double x, double L[3], double dL[3]) {
+  L[0] = 0.5 * x * (x - 1.0);
+  L[1] = 1.0 - x * x;
+  L[2] = 0.5 * x * (x + 1.0);
+  dL[0] = x - 0.5;
+  dL[1] = -2.0 * x;
+  dL[2] = x + 0.5;
+}
+
 // ---- shape functions --------------------------------------------------------
 template <int NPE, int DIM>
 struct Shape;
@@ -27,6 +45,7 @@ struct Shape;
 template <>
 struct Shape<4, 2> {  // Q1, nodes (-1,-1), (1,-1), (1,1), (-1,1)
   static constexpr int NQ = 4;
+  __device__ static double weight(int) { return 1.0; }  // 2-point rule: weights 2 * 0.5
   __device__ static void eval(int q, double N[4], double dN[4][2]) {
     const double x0 = 0.5 * gauss_x(q & 1), x2 = 0.5 * gauss_x(q >> 1);
     const double x1 = 0.5 - x0, x3 = 0.5 - x2, x4 = x0 + 0.5, x5 = x2 + 0.5;
@@ -41,6 +60,7 @@ struct Shape<4, 2> {  // Q1, nodes (-1,-1), (1,-1), (1,1), (-1,1)
 template <>
 struct Shape<8, 3> {  // Hex8, node signs per spaces.py:12057-12081
   static constexpr int NQ = 8;
+  __device__ static double weight(int) { return 1.0; }
   __device__ static void eval(int q, double N[8], double dN[8][3]) {
     const double xi[3] = {gauss_x(q & 1), gauss_x((q >> 1) & 1), gauss_x(q >> 2)};
 #pragma unroll
@@ -53,6 +73,29 @@ struct Shape<8, 3> {  // Hex8, node signs per spaces.py:12057-12081
       dN[a][0] = s0 * f1 * f2 * 0.125;
       dN[a][1] = f0 * s1 * f2 * 0.125;
       dN[a][2] = f0 * f1 * s2 * 0.125;
+    }
+  }
+};
+
+// Q2 (9-node) quad of the quadrilateral branch of spaces.fem_iso_line_quad_brick:
+// tensor products L_i(xi_0) L_j(xi_1) of the quadratic Lagrange polynomials in the
+// node order corners (-1,-1), (1,-1), (1,1), (-1,1), midsides (0,-1), (1,0), (0,1),
+// (-1,0), centre (0,0).  3 x 3 Gauss rule, xi_0 fastest (seeder.py:1046-1062).
+template <>
+struct Shape<9, 2> {
+  static constexpr int NQ = 9;
+  __device__ static double weight(int q) { return gauss3_w(q % 3) * gauss3_w(q / 3); }
+  __device__ static void eval(int q, double N[9], double dN[9][2]) {
+    double Lx[3], Ly[3], dLx[3], dLy[3];
+    lagrange3(gauss3_x(q % 3), Lx, dLx);
+    lagrange3(gauss3_x(q / 3), Ly, dLy);
+    constexpr int I[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1};  // xi_0 node index of node a
+    constexpr int J[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};  // xi_1 node index of node a
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      N[a] = Lx[I[a]] * Ly[J[a]];
+      dN[a][0] = dLx[I[a]] * Ly[J[a]];
+      dN[a][1] = Lx[I[a]] * dLy[J[a]];
     }
   }
 };
@@ -340,7 +383,7 @@ struct NeoHooke {
     // The reference differentiates psi with jacrev, whose derivative of log J
     // is 1/J for any J != 0; the closed forms below are that same derivative.
     const double c1 = mu - 0.5 * lam * (J * J - 1.0);
-    o.wdet = (T)det;
+    o.wdet = (T)(Shape<NPE, DIM>::weight(q) * det);
     o.c1 = (T)c1;
     o.c2 = (T)(lam * J * J);
 #pragma unroll
@@ -432,6 +475,55 @@ struct LineTraction {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int B = 0; B < 4; ++B) out[i][B] = 0;
+  }
+};
+
+// ---- dead-load traction on 3-node (quadratic) lines in 2D ----------------------------
+// Nodes at xi = -1, 1, 0 (line branch of spaces.fem_iso_line_quad_brick), 3-point rule:
+// r_ai = - sum_q w_q |dX/dxi(xi_q)| N_a(xi_q) t_i, zero tangent -- the residual of
+// isoparametric_surface_element_galerkin (models.py:1723-1850) with neumann_weak (:744-780).
+struct LineTraction3 {
+  static constexpr int NPE = 3, DIM = 2, D = 2, NQ = 3, NODAL = 0;
+  template <class T>
+  struct QP {
+    T wlen;
+  };
+  __device__ static void shape(double xi, double N[3], double dN[3]) {
+    N[0] = 0.5 * xi * (xi - 1.0);
+    N[1] = 0.5 * xi * (xi + 1.0);
+    N[2] = 1.0 - xi * xi;
+    dN[0] = xi - 0.5;
+    dN[1] = xi + 0.5;
+    dN[2] = -2.0 * xi;
+  }
+  template <class T>
+  __device__ static void qp(int q, const double (*x)[2], const double (*)[2], const double (*)[2],
+                            const ModelParams&, bool, QP<T>& o) {
+    double N[3], dN[3];
+    shape(gauss3_x(q), N, dN);
+    double tx = 0.0, ty = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      tx += x[a][0] * dN[a];
+      ty += x[a][1] * dN[a];
+    }
+    o.wlen = (T)(gauss3_w(q) * sqrt(tx * tx + ty * ty));
+  }
+  __device__ static void res_row(int a, const QP<double>* Q, const ModelParams& p, double out[2]) {
+    out[0] = out[1] = 0.0;
+    for (int q = 0; q < 3; ++q) {
+      double N[3], dN[3];
+      shape(gauss3_x(q), N, dN);
+      out[0] -= Q[q].wlen * N[a] * p.p[0];
+      out[1] -= Q[q].wlen * N[a] * p.p[1];
+    }
+  }
+  template <class T>
+  __device__ static void tan_rows(int, const QP<T>*, const ModelParams&, T out[2][6]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int B = 0; B < 6; ++B) out[i][B] = 0;
   }
 };
 

@@ -83,3 +83,16 @@ def in_plane(x, x_p1, normal):
 
 def in_planes(x_q, x_p1, normal):
     return in_plane(x_q, x_p1, normal)
+
+
+def elem_in_plane(x_nodes, x_p1, normal):
+    """geometry.py:1030-1045: every node of the element lies in the plane."""
+    return bool(np.all(in_planes(np.asarray(x_nodes), x_p1, normal)))
+
+
+def select_elements_in_plane(x_nodes, surface_elements, x_p1, normal):
+    """geometry.py:1048-1067: boolean mask of the surface elements lying in the plane."""
+    x_nodes = np.asarray(x_nodes, dtype=np.float64)
+    se = np.asarray(surface_elements)
+    inside = np.asarray(in_planes(x_nodes[se].reshape(-1, x_nodes.shape[1]), x_p1, normal))
+    return inside.reshape(se.shape).all(axis=1)

@@ -1,8 +1,10 @@
 """Gauss-Legendre rules (host), as autopdex.seeder (seeder.py:301-385, :1046-1062).
 
 gauss_legendre_nd(dimension, order) returns points on [-1, 1]^dim (2 p - 1)
-and weights 2 w, tensor ordered with xi_0 fastest.  The device kernels
-implement the 2-point rule (order 2 or 3); models reject other rules.
+and weights 2 w, tensor ordered with xi_0 fastest (the order-4|5 weights carry the
+reference's digits, seeder.py:387-393).  The device kernels implement the 2-point
+rule (order 2 or 3) and, for Q2 quads and 3-node lines, the 3-point rule (order 4
+or 5); models reject other rules.
 """
 import numpy as np
 
@@ -10,8 +12,8 @@ _GL01 = {
     1: ([0.5], [1.0]),
     2: ([0.21132486540518713, 0.7886751345948129], [0.5, 0.5]),
     3: ([0.21132486540518713, 0.7886751345948129], [0.5, 0.5]),
-    4: ([0.1127016653792583, 0.5, 0.8872983346207417], [0.2777777777777778, 0.4444444444444444, 0.2777777777777778]),
-    5: ([0.1127016653792583, 0.5, 0.8872983346207417], [0.2777777777777778, 0.4444444444444444, 0.2777777777777778]),
+    4: ([0.1127016653792583, 0.5, 0.8872983346207417], [0.2777777777777777, 0.44444444444444453, 0.2777777777777777]),
+    5: ([0.1127016653792583, 0.5, 0.8872983346207417], [0.2777777777777777, 0.44444444444444453, 0.2777777777777777]),
 }
 
 

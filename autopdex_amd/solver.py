@@ -120,7 +120,13 @@ def adaptive_load_stepping(dofs, settings, static_settings, multiplier_settings=
                            path_dependent=True, implicit_diff_mode=None, max_multiplier=1.0, min_increment=0.01,
                            max_increment=1.0, init_increment=0.2, max_load_steps=1000, target_num_newton_iter=7,
                            newton_tol=1e-10, **kwargs):
-    """solver.adaptive_load_stepping (solver.py:139-457) without implicit differentiation.
+    """solver.adaptive_load_stepping (solver.py:139-457).
+
+    implicit_diff_mode ('forward', 'backward' / 'reverse' or None) only decides, in the
+    reference, which JAX derivative rule the returned root carries (:203-293); the root itself is
+    the same.  Without JAX the derivatives of the returned root are computed explicitly by
+    autopdex_amd.implicit_diff (root_jvp, root_vjp, root_jvp2) with the device linear solve, so
+    the mode is validated and the solve runs unchanged.
 
     The increment control is the reference's (:296-379): multiplier += increment, one Newton
     solve; on divergence the step is undone and the increment halved, otherwise the increment
@@ -130,8 +136,11 @@ def adaptive_load_stepping(dofs, settings, static_settings, multiplier_settings=
     by a device copy.  Returns (dofs, multiplier, increment, load_step, settings, res_norm) like
     the reference's carry (its load_step is never incremented and stays 0)."""
     import torch
-    if implicit_diff_mode is not None:
-        raise NotImplementedError("implicit differentiation of load stepping is not part of the 'hip' backend")
+    if implicit_diff_mode not in (None, "forward", "backward", "reverse"):
+        raise ValueError("implicit_diff_mode must be None, 'forward', 'backward' or 'reverse'")
+    if implicit_diff_mode is not None and path_dependent:
+        raise NotImplementedError("implicit differentiation of a path-dependent load history is not implemented "
+                                  "(the reference differentiates the final root only with path_dependent=False)")
     _check_backend(static_settings)
     if static_settings["solver type"] not in ("newton", "damped newton"):
         raise NotImplementedError("adaptive load stepping works with solver types 'newton' and 'damped newton'")
@@ -173,6 +182,11 @@ def adaptive_load_stepping(dofs, settings, static_settings, multiplier_settings=
             increment = max_multiplier - multiplier
     sol = utility.reshape_as(u.cpu().numpy(), dofs)
     adaptive_load_stepping.last_steps = steps  # (multiplier, Newton iterations, diverged, Krylov iterations)
+    if implicit_diff_mode is not None:
+        # the reference's differentiable branch returns (dofs, (multiplier, increment, load_step,
+        # settings, res)) with settings at max_multiplier (:381-409)
+        settings = multiplier_settings(settings, max_multiplier)
+        return sol, (multiplier, increment, 0.0, settings, res)
     return sol, multiplier, increment, 0.0, settings, res
 
 

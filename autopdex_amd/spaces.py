@@ -1,10 +1,13 @@
-"""Solution spaces (host), as autopdex.spaces for the Q1 / Hex8 branches.
+"""Solution spaces (host), as autopdex.spaces for the 2- and 3-node line, Q1, Q2 and Hex8
+branches.
 
 fem_iso_line_quad_brick(x, xI, fI, settings, overwrite_diff, n_dim) evaluates
-sum_i N_i(x) fI_i with the Q1 (spaces.py:1913-1922) or Hex8
-(:12057-12081) shape functions.  Passing this function to
-models.mixed_reference_domain_potential selects the isoparametric Q1 / Hex8
-element kernels of the device library; the physical gradient those kernels
+sum_i N_i(x) fI_i with the line (spaces.py:353-380, nodes -1, 1, 0), Q1
+(:1913-1922), Q2 (the 9-node tensor Lagrange quad in the generator's
+square_indices order) or Hex8 (:12057-12081) shape functions.  Passing this
+function to models.mixed_reference_domain_potential or the isoparametric
+Galerkin element builders selects the isoparametric element kernels of the
+device library; the physical gradient those kernels
 use is (df/dxi) . (dX/dxi)^-1 as in the reference's custom JVP
 (:12162-12191).
 """
@@ -30,9 +33,26 @@ def _hex8(xi):
     return f[:, 0] * f[:, 1] * f[:, 2] / 8.0
 
 
+def _lagrange3(x):
+    return np.array([0.5 * x * (x - 1.0), 1.0 - x * x, 0.5 * x * (x + 1.0)])
+
+
+# Q2 node a = L_i(xi_0) L_j(xi_1) over the 1D nodes -1, 0, 1: corners, midsides (bottom,
+# right, top, left), centre -- the square_indices order of the quadrilateral branch
+_Q2_I = np.array([0, 2, 2, 0, 1, 2, 1, 0, 1])
+_Q2_J = np.array([0, 0, 2, 2, 0, 1, 2, 1, 1])
+
+
 def shape_functions(n_nodes, n_dim, xi):
+    xi = np.atleast_1d(np.asarray(xi, dtype=np.float64))
+    if n_dim == 1 and n_nodes == 2:
+        return np.array([0.5 - 0.5 * xi[0], 0.5 + 0.5 * xi[0]])
+    if n_dim == 1 and n_nodes == 3:
+        return _lagrange3(xi[0])[[0, 2, 1]]  # nodes -1, 1, 0
     if n_dim == 2 and n_nodes == 4:
         return _q1(xi)
+    if n_dim == 2 and n_nodes == 9:
+        return _lagrange3(xi[0])[_Q2_I] * _lagrange3(xi[1])[_Q2_J]
     if n_dim == 3 and n_nodes == 8:
         return _hex8(xi)
     raise NotImplementedError(f"{n_nodes}-node elements in {n_dim}D are not supported by autopdex_amd")

@@ -40,8 +40,16 @@ enum apx_model {
                                    examples/miscellaneous/short_example.py:23-33 */
   APX_MODEL_POISSON_VARK = 2,   /* 1/2 exp(N.g)|grad u|^2 - (N.f) u, nodal fields g, f       */
   APX_MODEL_NEO_HOOKE = 3,      /* Ciarlet Neo-Hooke psi(I + grad u), params (lambda, mu),
-                                   models.py:1122-1146; 2D = plane strain (F33 = 1)          */
-  APX_MODEL_LINE_TRACTION = 4   /* - t . u on 2-node boundary lines, params (t_x, t_y)       */
+                                   models.py:1122-1146; 2D = plane strain (F33 = 1).
+                                   Q1 quads / Hex8 bricks ('user potential',
+                                   models.py:1188-1269, 2-point rule) or Q2 9-node quads
+                                   ('user element' P : virt F of hyperelastic_steady_state_weak
+                                   :917-1000 in isoparametric_domain_element_galerkin
+                                   :1616-1720, 3 x 3 rule)                                    */
+  APX_MODEL_LINE_TRACTION = 4   /* - t . u on boundary lines in 2D, params (t_x, t_y): 2-node
+                                   lines (2-point rule) or 3-node lines (nodes xi = -1, 1, 0,
+                                   3-point rule; isoparametric_surface_element_galerkin
+                                   :1723-1850 with neumann_weak :744-780)                     */
 };
 
 enum apx_precision { APX_FP64 = 0, APX_FP32 = 1 };
@@ -59,7 +67,8 @@ int apx_plan_create(apx_plan** out, int dim, int64_t n_nodes, const double* coor
 
 /* One element set = one entry of settings['connectivity'] / static_settings['model']
  * (assembler.py:587-637 sums over sets).  conn_host: n_elems x nodes_per_elem
- * int64 row-major.  Returns the set index through *set_out.                 */
+ * int64 row-major, nodes_per_elem 2, 3 (lines), 4, 9 (quads) or 8 (bricks).
+ * Returns the set index through *set_out.                                   */
 int apx_plan_add_set(apx_plan* plan, const int64_t* conn_host, int64_t n_elems, int nodes_per_elem,
                      int model, const double* params, int n_params, int* set_out);
 

@@ -57,17 +57,51 @@ def line2_shape(xi):
     return np.array([(1 - xi) / 2, (1 + xi) / 2]), np.array([-0.5, 0.5])
 
 
+def lagrange3(x):
+    """Quadratic Lagrange polynomials on the nodes -1, 0, 1 and their derivatives."""
+    return (np.array([x * (x - 1) / 2, 1 - x * x, x * (x + 1) / 2]),
+            np.array([x - 0.5, -2 * x, x + 0.5]))
+
+
+def line3_shape(xi):
+    """3-node line of spaces.fem_iso_line_quad_brick (line branch, spaces.py:353-380): node
+    order xi = -1, 1, 0 (ends first, then the interior node)."""
+    L, dL = lagrange3(xi)
+    order = [0, 2, 1]
+    return L[order], dL[order]
+
+
+# quadrilateral branch of spaces.fem_iso_line_quad_brick: node a is the tensor product
+# L_i(xi_0) L_j(xi_1) with (i, j) from its square_indices ordering (corners
+# counter-clockwise from (-1,-1), then midsides bottom, right, top, left, then the centre);
+# i, j index the 1D nodes -1, 0, 1
+Q2_IJ = np.array([[0, 0], [2, 0], [2, 2], [0, 2], [1, 0], [2, 1], [1, 2], [0, 1], [1, 1]])
+
+
+def q2_shape(xi):
+    """9-node biquadratic quad in the reference's node order (Q2_IJ)."""
+    Lx, dLx = lagrange3(xi[0])
+    Ly, dLy = lagrange3(xi[1])
+    i, j = Q2_IJ[:, 0], Q2_IJ[:, 1]
+    return Lx[i] * Ly[j], np.stack([dLx[i] * Ly[j], Lx[i] * dLy[j]], axis=1)
+
+
 def shape(npe, dim, xi):
     if npe == 4 and dim == 2:
         return q1_shape(xi)
+    if npe == 9 and dim == 2:
+        return q2_shape(xi)
     if npe == 8 and dim == 3:
         return hex8_shape(xi)
     raise NotImplementedError((npe, dim))
 
 
-def _gauss(dim):
+def _gauss(dim, npe=None):
+    """The rule each element type is integrated with: 2 points per direction (order 2) for
+    Q1 / Hex8 / 2-node lines, 3 points (order 4 = 2 x order 2 as the reference's Q2 user
+    elements use) for Q2 quads and 3-node lines."""
     from .mesh import gauss_legendre_nd
-    return gauss_legendre_nd(dim, 2)
+    return gauss_legendre_nd(dim, 4 if npe in (3, 9) else 2)
 
 
 def geometry(xe, dN):
@@ -214,7 +248,7 @@ class NeoHooke:
 
     def _loop(self, xe, ue):
         E, npe, dim = xe.shape
-        xq, wq = _gauss(dim)
+        xq, wq = _gauss(dim, npe)
         for q in range(len(wq)):
             N, dN = shape(npe, dim, xq[q])
             det, G = geometry(xe, dN)
@@ -244,8 +278,10 @@ class NeoHooke:
 
 
 class LineTraction:
-    """Dead-load traction on 2-node boundary lines in 2D:
-    Pi = - sum_q w_q |dX/dxi| t . u(xi_q) (2-point Gauss), t = traction vector.
+    """Dead-load traction on boundary lines in 2D:
+    Pi = - sum_q w_q |dX/dxi| t . u(xi_q), t = traction vector; 2-node lines with the
+    2-point rule, 3-node lines with the 3-point rule (isoparametric_surface_element_galerkin,
+    models.py:1723-1850, with neumann_weak :744-780: -virt_u . t times w |G_1 x E_3|).
     Residual r_ai = - sum_q w_q |dX/dxi| N_a t_i; tangent zero.
     """
 
@@ -254,10 +290,10 @@ class LineTraction:
 
     def residual(self, xe, ue, nodal=None):
         E, npe, dim = xe.shape
-        xq, wq = _gauss(1)
+        xq, wq = _gauss(1, npe)
         out = np.zeros((E, npe, dim))
         for q in range(len(wq)):
-            N, dN = line2_shape(xq[q])
+            N, dN = line2_shape(xq[q]) if npe == 2 else line3_shape(xq[q])
             t = np.einsum("a,eai->ei", dN, xe)
             L = np.sqrt(np.sum(t * t, -1))
             out -= (wq[q] * L)[:, None, None] * N[None, :, None] * self.t[None, None, :]

@@ -130,23 +130,31 @@ def dof_select(nodes_mask, fields):
 # --- Gauss-Legendre (autopdex/seeder.py) --------------------------------------
 
 def gauss_legendre_nd(dimension, order=2):
-    """seeder.py:1046-1062 + :370-385 + tensor_product_rule :324-367 (order 2|3 only).
+    """seeder.py:1046-1062 + :370-393 + tensor_product_rule :324-367 (orders 2..5).
 
-    Points 2*p-1 with p in {0.21132486540518713, 0.7886751345948129}, weights
-    2*0.5; xi_0 varies fastest, then xi_1, then xi_2.
+    Order 2|3: points 2*p-1 with p in {0.21132486540518713, 0.7886751345948129},
+    weights 2*0.5.  Order 4|5: p in {0.1127016653792583, 0.5, 0.8872983346207417},
+    weights 2*{0.2777777777777777, 0.44444444444444453, 0.2777777777777777} (the
+    reference's digits).  xi_0 varies fastest, then xi_1, then xi_2.
     """
-    assert order in (2, 3)
-    r = 2 * np.array([0.21132486540518713, 0.7886751345948129]) - 1
-    w = 2 * np.array([0.5, 0.5])
+    if order in (2, 3):
+        r = 2 * np.array([0.21132486540518713, 0.7886751345948129]) - 1
+        w = 2 * np.array([0.5, 0.5])
+    elif order in (4, 5):
+        r = 2 * np.array([0.1127016653792583, 0.5, 0.8872983346207417]) - 1
+        w = 2 * np.array([0.2777777777777777, 0.44444444444444453, 0.2777777777777777])
+    else:
+        raise ValueError(order)
+    n = r.size
     if dimension == 1:
         return r, w
     if dimension == 2:
-        xi = np.tile(r, 2)
-        yi = np.repeat(r, 2)
+        xi = np.tile(r, n)
+        yi = np.repeat(r, n)
         return np.stack([xi, yi], axis=1), np.outer(w, w).ravel()
     if dimension == 3:
-        xi = np.tile(r, 4)
-        yi = np.tile(np.repeat(r, 2), 2)
-        zi = np.repeat(r, 4)
+        xi = np.tile(r, n * n)
+        yi = np.tile(np.repeat(r, n), n)
+        zi = np.repeat(r, n * n)
         return np.stack([xi, yi, zi], axis=1), np.outer(np.outer(w, w).ravel(), w).ravel()
     raise ValueError(dimension)

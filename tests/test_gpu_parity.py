@@ -38,11 +38,13 @@ CASES = [
     ("poisson_rect_tiles_t", P.poisson_rect, (70, 5)),
     ("poisson_vark_q1_tiles", P.poisson_vark, (70, 5)),
     ("neo_hooke_plane_strain_tiles", P.tapered_panel, (66,)),
+    # 'user element' mode: Q2 Neo-Hooke and 3-node traction lines on an unstructured mesh
+    ("cook_membrane_q2", P.cook_membrane_q2, ()),
 ]
 # bit 0: owner-tile residual kernels in use; bit 1: owner-tile tangent too
 STRUCTURED = {"poisson_q1": 3, "poisson_vark_q1": 3, "neo_hooke_hex8": 0, "neo_hooke_plane_strain_traction": 1,
               "poisson_q1_tiles": 3, "poisson_rect_tiles": 3, "poisson_rect_tiles_t": 3, "poisson_vark_q1_tiles": 3,
-              "neo_hooke_plane_strain_tiles": 1}
+              "neo_hooke_plane_strain_tiles": 1, "cook_membrane_q2": 0}
 
 
 @pytest.mark.parametrize("name,builder,args", CASES, ids=[c[0] for c in CASES])
@@ -217,6 +219,77 @@ def test_adaptive_load_stepping_matches_oracle(torch_gpu):
     assert [(round(a, 12), b, c) for a, b, c, _ in dev_steps] == [(round(a, 12), b, c) for a, b, c in ref_steps]
     assert m == m_ref == 1.0
     assert np.linalg.norm(sol.ravel() - u_ref) / np.linalg.norm(u_ref) < 1e-8
+
+
+def test_cook_membrane_q2_user_elements_reach_reference_golden(torch_gpu):
+    """The reference's Q2 user-element test (tests/test_user_elem_impl_diff_and_adaptive_load_step.py):
+    Cook's membrane, 12 Q2 elements, plane-strain Neo-Hooke E = 100, nu = 0.3, traction
+    (0, 20 m) on 3-node lines, adaptive load stepping to m = 1 with newton_tol 1e-8.  The
+    reference asserts allclose(dofs . dofs, 19390.35027108); the device run must meet that, agree
+    with the value to 1e-10, and follow the oracle's multiplier sequence and Newton counts
+    (the reference solves each Newton step directly; here Jacobi-PCG to 1e-12)."""
+    from autopdex_amd import solver
+    fx = P.cook_membrane_fixture()
+    golden = fx["golden"]["dofs_dot_dofs"]
+    dofs, settings, static, o = P.cook_membrane_q2()
+    static = dict(static, **{"linear tolerance": 1e-12})
+
+    def multiplier_settings(s, m):
+        s["load multiplier"] = m * fx["q_0"]
+        return s
+
+    sol, m, inc, _, _, res = solver.adaptive_load_stepping(dofs, settings, static, multiplier_settings, False, None,
+                                                           newton_tol=fx["newton_tol"])
+    steps = solver.adaptive_load_stepping.last_steps
+    u_ref, ref_steps = P.oracle_cook_membrane()
+    val = float(sol.ravel() @ sol.ravel())
+    print(f"[case] cook membrane dofs.dofs {val!r} golden {golden!r} steps {[(round(a, 4), b) for a, b, c, _ in steps]}",
+          flush=True)
+    assert np.isclose(val, golden, rtol=fx["golden_rtol"], atol=fx["golden_atol"])
+    assert abs(val / golden - 1.0) < 1e-10
+    assert m == 1.0
+    assert [(round(a, 12), b, c) for a, b, c, _ in steps] == [(round(a, 12), b, c) for a, b, c in ref_steps]
+    assert np.linalg.norm(sol.ravel() - u_ref) / np.linalg.norm(u_ref) < 1e-9
+
+
+def test_cook_membrane_sensitivities_match_reference(torch_gpu):
+    """autopdex_amd.implicit_diff on the device for the reference's sensitivities of u.u
+    (tests/test_user_elem_impl_diff_and_adaptive_load_step.py:151-172): first order in forward
+    (root_jvp) and reverse (root_vjp) mode to 1e-9 of the reference's values, second order
+    (root_jvp2, forward over forward) to 1e-7; and against the oracle's rules."""
+    from autopdex_amd import implicit_diff, solver
+    fx = P.cook_membrane_fixture()
+    g = fx["golden"]
+    dofs, settings, static, o = P.cook_membrane_q2()
+    static = dict(static, **{"linear tolerance": 1e-12, "sensitivity solver backend": "hip"})
+
+    def multiplier_settings(s, m):
+        s["load multiplier"] = m * fx["q_0"]
+        return s
+
+    sol, (m, inc, _, settings, res) = solver.adaptive_load_stepping(
+        dofs, settings, static, multiplier_settings, False, "forward", newton_tol=fx["newton_tol"])
+    u = sol.ravel()
+    keys = ["youngs modulus", "poisson ratio"]
+    first = [2 * u @ implicit_diff.root_jvp(sol, settings, static, k).ravel() for k in keys]
+    rev = implicit_diff.root_vjp(sol, settings, static, 2 * sol, keys)
+    ref1 = [g["d_dE"], g["d_dnu"]]
+    print(f"[case] cook sensitivities forward {first} reverse {rev}", flush=True)
+    for k, key in enumerate(keys):
+        assert abs(first[k] / ref1[k] - 1) < 1e-9 and abs(rev[key] / ref1[k] - 1) < 1e-9
+    second = {}
+    for i, j, name in [(0, 0, "d2_dE2"), (0, 1, "d2_dE_dnu"), (1, 1, "d2_dnu2")]:
+        u_i, u_j, u_ij = implicit_diff.root_jvp2(sol, settings, static, keys[i], keys[j])
+        second[name] = 2 * (u_i.ravel() @ u_j.ravel() + u @ u_ij.ravel())
+    print(f"[case] cook second order {second}", flush=True)
+    for name, v in second.items():
+        assert np.isclose(v, g[name], rtol=1e-5, atol=1e-8)  # the reference's allclose
+        assert abs(v / g[name] - 1) < 1e-7
+    u_ref, _ = P.oracle_cook_membrane()
+    o1, _, o2 = P.oracle_cook_sensitivities(u_ref, eps=1e-5)
+    assert np.allclose(first, o1, rtol=1e-9, atol=0)
+    assert np.allclose([second["d2_dE2"], second["d2_dE_dnu"], second["d2_dnu2"]], [o2[0, 0], o2[0, 1], o2[1, 1]],
+                       rtol=1e-7, atol=0)
 
 
 @pytest.mark.parametrize("name,builder,args", CASES[:4], ids=[c[0] for c in CASES[:4]])
