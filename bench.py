@@ -26,7 +26,7 @@ host: residual + tangent assembly of a 1/16 element slab (scaled x16) plus a
 fixed number of scipy Jacobi-PCG iterations on the full-size reduced matrix
 (scaled to the plain method's iteration count).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--methods plain,mixed,learned_mixed]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--methods plain,mixed,learned,learned_mixed]
                 [--headline learned_mixed] [--train-steps 1500] [--no-cpu]
 """
 import argparse
@@ -46,8 +46,8 @@ HBM_PEAK = 8.0e12
 # HBM bytes per executing launch at n = 2048, from `rocprofv3 --pmc FETCH_SIZE` and
 # `--pmc WRITE_SIZE` runs of `python bench.py --probe`, scaled by the counters'
 # reading of a 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000).
-PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double>", 2048): 674.2e6, ("k_pcg_spmv<float>", 2048): 409.6e6,
-                     ("k_conv3x3<16, 16, false, 1>", 2048): 287.8e6}
+PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double>", 2048): 660.2e6, ("k_pcg_spmv<float>", 2048): 408.5e6,
+                     ("k_conv3x3_ws<16, 16, 4>", 2048): 295.8e6}
 
 
 def build_problem(n):
@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--methods", default="plain,mixed,learned_mixed")
+    ap.add_argument("--methods", default="plain,mixed,learned,learned_mixed")
     ap.add_argument("--headline", default="learned_mixed")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--seed", type=int, default=0, help="seed of the preconditioner's training data")
@@ -269,7 +269,7 @@ def main():
     if H.startswith("learned"):
         # the heaviest single kernel of flexible CG with the network: the level-0 16 -> 16 convolution
         dom = kernels["unet_conv_e0"]
-        dom_name = "k_conv3x3<16, 16, false, 1>"
+        dom_name = "k_conv3x3_ws<16, 16, 4>"
     else:
         prec_tag = "f64" if H == "plain" else "f32"
         dom = kernels[f"pcg_spmv_{prec_tag}"]
