@@ -105,6 +105,18 @@ void fill(T* p, int64_t n, T v, hipStream_t s) {
 template void fill<double>(double*, int64_t, double, hipStream_t);
 template void fill<float>(float*, int64_t, float, hipStream_t);
 
+__global__ void k_round_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];  // round to nearest
+}
+
+void round_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_round_to_f32, dim3(blocks), dim3(256), 0, s, src, dst, n);
+  APX_LAUNCH_CHECK();
+}
+
 template <class M, class T, int MODE>
 static void run_set(Plan& P, ElemSet& S, const double* u, double* r, void* vals, hipStream_t st) {
   // whole elements per block: every thread's element slot le = threadIdx.x / NPE is < EPB

@@ -6,6 +6,8 @@ namespace apx {
 
 template <class T>
 void fill(T* p, int64_t n, T v, hipStream_t s);
+// dst = src rounded to FP32 (nearest), element by element.
+void round_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 
 void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st);
 void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st);
