@@ -180,10 +180,17 @@ def test_neo_hooke_cube_load_steps_match_oracle(torch_gpu):
         u_dev, u_ref = sol, ref
 
 
-def test_tapered_panel_traction_matches_oracle(torch_gpu):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tapered_panel_traction_matches_oracle(torch_gpu, precision):
+    """Plane-strain Neo-Hooke with traction, Newton to 1e-9 (absolute).  'mixed' re-assembles the
+    tangent every Newton step in FP64, rounds it to FP32 and solves in FP32 to 1e-5; the FP64
+    Newton residual decides, so the solution meets the same bar."""
     from autopdex_amd import solver
     dofs, settings, static, o = P.tapered_panel(8, multiplier=1.0)
-    static = dict(static, **{"linear tolerance": 1e-12})
+    if precision == "fp64":
+        static = dict(static, **{"linear tolerance": 1e-12})
+    else:
+        static = dict(static, **{"linear tolerance": 1e-5, "hip precision": "mixed"})
     sol, infos = solver.solver(dofs, settings, static, newton_tol=1e-9)
     ref, rinfo = P.oracle_newton(o, tol=1e-9)
     assert infos[2] is False and rinfo[2] is False
