@@ -29,7 +29,8 @@ class NewtonOptions(C.Structure):
     _fields_ = [("newton_tol", C.c_double), ("relative", C.c_int), ("maxiter", C.c_int),
                 ("damping", C.c_double), ("precision", C.c_int), ("precond", C.c_int),
                 ("lin_tol", C.c_double), ("lin_atol", C.c_double), ("lin_maxiter", C.c_int),
-                ("reuse_tangent", C.c_int), ("forcing", C.c_int), ("reliable_delta", C.c_double)]
+                ("reuse_tangent", C.c_int), ("forcing", C.c_int), ("reliable_delta", C.c_double),
+                ("fp32_element_arith", C.c_int)]
 
 
 class NewtonStats(C.Structure):

@@ -57,10 +57,11 @@ struct NewtonCtx {
   // needed 76 iterations over 6 Newton steps on the FP32-assembled matrix and 50 over 5 on the
   // rounded one (Jacobi-PCG: 20692 and 13456); one FP64 assembly plus the rounding cost far less
   // than one network application.
+  bool fp32_arith = false;  // FP32 tangent: element arithmetic in FP32 instead of rounding the FP64 one
   void tangent(void* vals, void* inv, int prec, int precond) {
     auto t0 = std::chrono::steady_clock::now();
     EventPair e = timing_begin(P, st);
-    if (prec == APX_FP32) {
+    if (prec == APX_FP32 && !fp32_arith) {
       if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
       assemble_tangent(P, u, P.vals64.p, APX_FP64, st);
       round_to_f32(P.vals64.p, (float*)vals, P.sell_total, st);
@@ -81,9 +82,12 @@ struct NewtonCtx {
   void finish(int prec) {
     APX_HIP(hipStreamSynchronize(st));
     timing_collect(P, "residual", ev_res, residual_bytes(P), ev_res.size());
-    timing_collect(P, prec == APX_FP64 ? "tangent_f64" : "tangent_f64_to_f32", ev_tan,
-                   prec == APX_FP64 ? tangent_bytes(P, APX_FP64) : tangent_bytes(P, APX_FP64) + P.sell_total * 12,
-                   ev_tan.size());
+    if (prec == APX_FP64)
+      timing_collect(P, "tangent_f64", ev_tan, tangent_bytes(P, APX_FP64), ev_tan.size());
+    else if (fp32_arith)
+      timing_collect(P, "tangent_f32", ev_tan, tangent_bytes(P, APX_FP32), ev_tan.size());
+    else
+      timing_collect(P, "tangent_f64_to_f32", ev_tan, tangent_bytes(P, APX_FP64) + P.sell_total * 12, ev_tan.size());
   }
 };
 
@@ -245,6 +249,7 @@ void newton(Plan& P, double* u, const double* dvals, const apx_newton_options& o
   if (P.r_full.n < (size_t)P.n_dofs) P.r_full.alloc(P.n_dofs);
   s = apx_newton_stats{};
   NewtonCtx C{P, u, P.r_full.p, st, {}, {}, s};
+  C.fp32_arith = o.fp32_element_arith != 0;
   if (o.precision == APX_FP32 && o.reuse_tangent && o.reliable_delta > 0.0 && o.precond != APX_PRECOND_LEARNED) {
     APX_REQUIRE(o.damping == 1.0, APX_ERR_ARG, "reliable-update refinement needs damping 1");
     newton_reliable(C, dvals, o);

@@ -94,6 +94,10 @@ def newton_options(static_settings, newton_tol=1e-8, maxiter=30, damping=1.0, re
     o.reuse_tangent = 1 if reuse_tangent else 0
     o.forcing = 1 if _opt(static_settings, "linear forcing", False) else 0
     o.reliable_delta = float(_opt(static_settings, "reliable delta", 0.1)) if prec == "mixed" else 0.0
+    arith = _opt(static_settings, "hip tangent arithmetic", "fp64")
+    if arith not in ("fp64", "fp32"):
+        raise ValueError("'hip tangent arithmetic' must be 'fp64' (FP64 tangent rounded to FP32 in 'mixed') or 'fp32'")
+    o.fp32_element_arith = 1 if arith == "fp32" else 0
     return o
 
 

@@ -4,12 +4,15 @@ Workload (one "step"): a full Newton solve on one GPU to a 1e-10 FP64
 relative residual ||r_free(u)||/||r_free(u0)||, inputs resident in HBM, for
 each method:
   plain         = FP64 assembly + FP64 Jacobi-PCG (the drop-in method)
-  mixed         = FP32 tangent + one FP32 Jacobi-PCG re-anchored to the FP64
-                  residual (reliable-update refinement)
+  mixed         = FP64 tangent rounded once to FP32 + one FP32 Jacobi-PCG
+                  re-anchored to the FP64 residual (reliable-update refinement)
   learned       = FP64 tangent + flexible CG with the U-Net preconditioner
                   (bf16 convolutions on MFMA, FP32 accumulation)
-  learned_mixed = FP32 tangent + FP32-matrix flexible CG with the U-Net, each
-                  inner solve re-anchored by the FP64 Newton residual
+  learned_mixed = FP64 tangent rounded once to FP32 + FP32-matrix flexible CG
+                  with the U-Net, each inner solve re-anchored by the FP64
+                  Newton residual
+  learned_mixed_fp32_elements = learned_mixed with the tangent's element
+                  arithmetic in FP32 (FP32 accumulation) instead of rounding
 The network is trained on the GPU before the timed steps (learned.fit: pairs
 from plain Jacobi-PCG solves of random right-hand sides, A-norm loss); its
 setup time is reported in `learned_setup` and is not part of any step.
@@ -26,7 +29,8 @@ host: residual + tangent assembly of a 1/16 element slab (scaled x16) plus a
 fixed number of scipy Jacobi-PCG iterations on the full-size reduced matrix
 (scaled to the plain method's iteration count).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--methods plain,mixed,learned,learned_mixed]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048]
+                [--methods plain,mixed,learned,learned_mixed,learned_mixed_fp32_elements]
                 [--headline learned_mixed] [--train-steps 1500] [--no-cpu]
 """
 import argparse
@@ -81,13 +85,15 @@ def options(static, method):
     elif method == "mixed":
         s["hip precision"] = "mixed"
         s["reliable delta"] = float(os.environ.get("APX_RELIABLE_DELTA", "0.1"))
-    elif method in ("learned", "learned_mixed"):
+    elif method in ("learned", "learned_mixed", "learned_mixed_fp32_elements"):
         s["type of preconditioner"] = "learned"
         s["linear tolerance"] = 1e-10
         s["linear maxiter"] = 3000
-        if method == "learned_mixed":
+        if method != "learned":
             s["hip precision"] = "mixed"
             s["reliable delta"] = float(os.environ.get("APX_LEARNED_DELTA", "1e-2"))
+        if method == "learned_mixed_fp32_elements":
+            s["hip tangent arithmetic"] = "fp32"
     else:
         raise ValueError(method)
     return solver.newton_options(s, newton_tol=1e-10, maxiter=100)
@@ -149,8 +155,8 @@ def cpu_baseline(prob, coords, elems, mask, plain_iters, vals64_cpu):
 def kernel_table(prob):
     out = {}
     for name in ("pcg_spmv_f64", "pcg_update_f64", "pcg_spmv_f32", "pcg_update_f32", "residual", "tangent_f64",
-                 "tangent_f64_to_f32", "fcg_spmv_f64", "fcg_spmv_f32", "fcg_precond", "unet_conv_e0",
-                 "unet_forward"):
+                 "tangent_f64_to_f32", "tangent_f32", "fcg_spmv_f64", "fcg_spmv_f32", "fcg_precond",
+                 "unet_conv_e0", "unet_forward"):
         ms, cnt, by = timer(prob, name)
         if cnt:
             out[name] = {"avg_ms": ms, "count": cnt, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9 if by else None,
@@ -203,7 +209,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--methods", default="plain,mixed,learned,learned_mixed")
+    ap.add_argument("--methods", default="plain,mixed,learned,learned_mixed,learned_mixed_fp32_elements")
     ap.add_argument("--headline", default="learned_mixed")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--seed", type=int, default=0, help="seed of the preconditioner's training data")
@@ -276,7 +282,8 @@ def main():
         dom = kernels[f"pcg_spmv_{prec_tag}"]
         dom_name = f"k_pcg_spmv<{'double' if prec_tag == 'f64' else 'float'}>"
     res = kernels.get("residual", {})
-    dtypes = {"plain": "f64", "mixed": "f32+f64", "learned": "bf16+f64", "learned_mixed": "bf16+f32+f64"}
+    dtypes = {"plain": "f64", "mixed": "f32+f64", "learned": "bf16+f64", "learned_mixed": "bf16+f32+f64",
+              "learned_mixed_fp32_elements": "bf16+f32+f64"}
     out = {
         "metric": METRIC, "value": T[H], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": T[H] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -295,7 +302,8 @@ def main():
                      "residual_frac_hbm": res.get("frac_hbm"),
                      "dofs_assembled_per_s": prob.n_dofs / (res["avg_ms"] * 1e-3) if res else None,
                      "tangent_f64_ms": kernels.get("tangent_f64", {}).get("avg_ms"),
-                     "tangent_f64_to_f32_ms": kernels.get("tangent_f64_to_f32", {}).get("avg_ms")},
+                     "tangent_f64_to_f32_ms": kernels.get("tangent_f64_to_f32", {}).get("avg_ms"),
+                     "tangent_f32_ms": kernels.get("tangent_f32", {}).get("avg_ms")},
         "kernels": kernels,
         "network": ({"forward_ms": kernels["unet_forward"]["avg_ms"], "forward_GBps": kernels["unet_forward"]["GBps"],
                      "forward_bytes": kernels["unet_forward"]["bytes"]} if "unet_forward" in kernels else None),

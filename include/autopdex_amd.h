@@ -172,6 +172,9 @@ typedef struct apx_newton_options {
                                whose residual is re-anchored to the FP64 residual
                                whenever it drops by this factor (reliable updates);
                                each anchor is a Newton iteration                      */
+  int fp32_element_arith;   /* precision FP32 only.  0: the tangent is assembled in FP64
+                               and rounded once to FP32 (default); 1: the element
+                               arithmetic of the tangent runs in FP32 (FP32 accumulation)  */
 } apx_newton_options;
 
 typedef struct apx_newton_stats {

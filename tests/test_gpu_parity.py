@@ -150,11 +150,15 @@ def test_config1_200x200_matches_superlu(torch_gpu):
     assert infos[1] / infos.details["initial_residual"] <= 1e-10
 
 
-def test_mixed_precision_refinement_reaches_fp64(torch_gpu):
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+def test_mixed_precision_refinement_reaches_fp64(torch_gpu, arith):
+    """FP32 PCG with FP64 reliable updates reaches the FP64 solution, with the FP32 tangent
+    either the FP64 one rounded (default) or assembled with FP32 element arithmetic."""
     from autopdex_amd import solver
     dofs, settings, static, o = P.poisson_source(128, dict_dofs=False)
     s64 = dict(static, **{"linear tolerance": 1e-12, "newton tolerance mode": "relative"})
-    s32 = dict(static, **{"linear tolerance": 1e-4, "newton tolerance mode": "relative", "hip precision": "mixed"})
+    s32 = dict(static, **{"linear tolerance": 1e-4, "newton tolerance mode": "relative", "hip precision": "mixed",
+                          "hip tangent arithmetic": arith})
     u64, i64 = solver.solver(dofs, settings, s64, newton_tol=1e-10)
     u32, i32 = solver.solver(dofs, settings, s32, newton_tol=1e-10)
     assert i32[2] is False
