@@ -235,8 +235,10 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
       ev_spmv.push_back(e1);
     }
   };
-  // host polling: batches of kBatch iterations, flags read one batch late
-  constexpr int kBatch = 2;
+  // host polling: batches of kBatch iterations, flags read one batch late.  One iteration per
+  // batch: an iteration is ~1.5 ms of device work against ~0.3 ms to enqueue it, and every
+  // iteration queued past convergence costs ~70 early-exit launches
+  constexpr int kBatch = 1;
   volatile char* pin = (volatile char*)P.pinned + 512;  // slot s: KState copy at byte 512 + 128 s
   hipEvent_t evs[2];
   APX_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
