@@ -140,7 +140,7 @@ __device__ __forceinline__ void bf8_unpack(const u32x4& q, float f[8]) {
 __device__ __forceinline__ u32x4 bf8_pack(const float f[8]) {
   u32x4 q;
 #pragma unroll
-  for (int e = 0; e < 4; ++e) q[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+  for (int e = 0; e < 4; ++e) q[e] = f2bf2(f[2 * e], f[2 * e + 1]);
   return q;
 }
 

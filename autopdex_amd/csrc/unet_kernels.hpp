@@ -27,12 +27,13 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  // round to nearest even; NaN kept quiet
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// FP32 -> bf16 with round to nearest even (NaN stays a quiet NaN): the gfx950 conversion
+// instruction (v_cvt_pk_bf16_f32), one value or a packed pair (low half = a).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2_t));
 }
 
 constexpr int kTileW = 64;  // output pixels per tile row (4 MFMA row tiles)
@@ -307,7 +308,20 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
 
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + TR - 1) / TR;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
-  // load slot i: 16-lane group g = i / 16 takes chunk g % CH8 of pixels 16 (g / CH8) + i % 16
+  // load slot i: 16-lane group g = i / 16 takes chunk g % CH8 of pixels 16 (g / CH8) + i % 16.
+  // A slot's place in the staged tile does not change from tile to tile: its row and column,
+  // chunk and LDS offset are computed once (row -> a large negative value for unused slots).
+  int s_row[RC], s_col[RC], s_h[RC], s_lds[RC];
+#pragma unroll
+  for (int r = 0; r < RC; ++r) {
+    const int i = threadIdx.x + r * kConvThreads;
+    const int g = i >> 4, h = g % CH8, pix = (g / CH8) * 16 + (i & 15);
+    const bool used = i < NL && pix < NP;
+    s_row[r] = used ? pix / TWP : -(1 << 20);
+    s_col[r] = pix % TWP;
+    s_h[r] = h;
+    s_lds[r] = used ? ((2 * h) * PP + pix) * 4 : -1;
+  }
   u32x4 qs[RC];
   auto load_tile = [&](int64_t tile) {
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
@@ -316,11 +330,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
     const int64_t img_off = (int64_t)img * a.H * a.W * CI;
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      const int i = threadIdx.x + r * kConvThreads;
-      const int g = i >> 4, h = g % CH8, pix = (g / CH8) * 16 + (i & 15);
-      const int yy = ty0 - 1 + pix / TWP, xx = tx0 - 1 + pix % TWP;
+      const int h = s_h[r];
+      const int yy = ty0 - 1 + s_row[r], xx = tx0 - 1 + s_col[r];
       u32x4 q = u32x4{0u, 0u, 0u, 0u};
-      if (i < NL && pix < NP && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+      if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
         const int64_t o = img_off + ((int64_t)yy * a.W + xx) * CI + 8 * h;
         q = *(const u32x4*)(a.x + o);
         if (a.xmask) {
@@ -346,11 +359,9 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
     __syncthreads();  // every wavefront is done reading the previous tile
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      const int i = threadIdx.x + r * kConvThreads;
-      const int g = i >> 4, h = g % CH8, pix = (g / CH8) * 16 + (i & 15);
-      if (i < NL && pix < NP) {
-        *(uint2*)(sx + ((2 * h) * PP + pix) * 4) = uint2{qs[r][0], qs[r][1]};
-        *(uint2*)(sx + ((2 * h + 1) * PP + pix) * 4) = uint2{qs[r][2], qs[r][3]};
+      if (s_lds[r] >= 0) {
+        *(uint2*)(sx + s_lds[r]) = uint2{qs[r][0], qs[r][1]};
+        *(uint2*)(sx + s_lds[r] + PP * 4) = uint2{qs[r][2], qs[r][3]};
       }
     }
     __syncthreads();
@@ -386,15 +397,13 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
           uint16_t* dst = (uint16_t*)a.y + (((int64_t)img * a.H + yy) * a.W + xx) * CO + 4 * lq;
 #pragma unroll
           for (int n = 0; n < NO; ++n) {
-            uint16_t o[4];
+            float v[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              float v = acc[r][n][j] + bb[n][j];
-              if (a.relu) v = fmaxf(v, 0.f);
-              o[j] = f2bf(v);
+              v[j] = acc[r][n][j] + bb[n][j];
+              if (a.relu) v[j] = fmaxf(v[j], 0.f);
             }
-            *(uint2*)(dst + 16 * n) =
-                uint2{(uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16)};
+            *(uint2*)(dst + 16 * n) = uint2{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])};
           }
         }
       }
