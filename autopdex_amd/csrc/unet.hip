@@ -64,6 +64,8 @@ void launch_conv_t(const ConvArgs& a, hipStream_t st) {
 template <int CI, int CO, int TR>
 void launch_conv_ws_t(const ConvArgs& a, hipStream_t st) {
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + TR - 1) / TR);
+  APX_REQUIRE(tiles < (int64_t(1) << 30) && (int64_t)(TR + 2) * a.W * CI < (int64_t(1) << 31), APX_ERR_UNSUPPORTED,
+              "convolution grid too large for 32-bit tile indices");
   static const int64_t cap = resident_blocks(k_conv3x3_ws<CI, CO, TR>);
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
   hipLaunchKernelGGL((k_conv3x3_ws<CI, CO, TR>), dim3(g), dim3(kConvThreads), 0, st, a);

@@ -307,11 +307,13 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
     for (int j = 0; j < 4; ++j) bb[n][j] = a.bias ? a.bias[16 * n + 4 * lq + j] : 0.f;
 
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + TR - 1) / TR;
-  const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
+  const int tiles_img = tiles_x * tiles_y;
+  const int ntiles = a.B * tiles_img;  // the launcher checks that tile indices fit in 32 bits
   // load slot i: 16-lane group g = i / 16 takes chunk g % CH8 of pixels 16 (g / CH8) + i % 16.
   // A slot's place in the staged tile does not change from tile to tile: its row and column,
   // chunk and LDS offset are computed once (row -> a large negative value for unused slots).
-  int s_row[RC], s_col[RC], s_h[RC], s_lds[RC];
+  // s_off: the slot's element offset from the tile's staged origin (row ty0 - 1, column tx0 - 1)
+  int s_row[RC], s_col[RC], s_off[RC], s_lds[RC];
 #pragma unroll
   for (int r = 0; r < RC; ++r) {
     const int i = threadIdx.x + r * kConvThreads;
@@ -319,22 +321,22 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
     const bool used = i < NL && pix < NP;
     s_row[r] = used ? pix / TWP : -(1 << 20);
     s_col[r] = pix % TWP;
-    s_h[r] = h;
+    s_off[r] = ((pix / TWP) * a.W + pix % TWP) * CI + 8 * h;
     s_lds[r] = used ? ((2 * h) * PP + pix) * 4 : -1;
   }
   u32x4 qs[RC];
-  auto load_tile = [&](int64_t tile) {
-    const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
-    const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
-    const int ty0 = (trem / tiles_x) * TR, tx0 = (trem % tiles_x) * kTileW;
-    const int64_t img_off = (int64_t)img * a.H * a.W * CI;
+  auto load_tile = [&](int tile) {
+    const int img = tile / tiles_img, trem = tile - img * tiles_img;
+    const int tyi = trem / tiles_x;
+    const int ty0 = tyi * TR, tx0 = (trem - tyi * tiles_x) * kTileW;
+    // element offset of the staged origin (may be negative; every loaded slot lands inside the image)
+    const int64_t origin = ((int64_t)img * a.H * a.W + (int64_t)(ty0 - 1) * a.W + (tx0 - 1)) * CI;
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      const int h = s_h[r];
       const int yy = ty0 - 1 + s_row[r], xx = tx0 - 1 + s_col[r];
       u32x4 q = u32x4{0u, 0u, 0u, 0u};
-      if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
-        const int64_t o = img_off + ((int64_t)yy * a.W + xx) * CI + 8 * h;
+      if ((unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W) {
+        const int64_t o = origin + s_off[r];
         q = *(const u32x4*)(a.x + o);
         if (a.xmask) {
           const u32x4 m = *(const u32x4*)(a.xmask + o);
@@ -350,12 +352,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
       qs[r] = q;
     }
   };
-  const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
+  const int tile0 = (int)xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
   if (tile0 < ntiles) load_tile(tile0);
-  for (int64_t tile = tile0; tile < ntiles; tile += gridDim.x) {
-    const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
-    const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
-    const int ty0 = (trem / tiles_x) * TR, tx0 = (trem % tiles_x) * kTileW;
+  for (int tile = tile0; tile < ntiles; tile += gridDim.x) {
+    const int img = tile / tiles_img, trem = tile - img * tiles_img;
+    const int tyi = trem / tiles_x;
+    const int ty0 = tyi * TR, tx0 = (trem - tyi * tiles_x) * kTileW;
     __syncthreads();  // every wavefront is done reading the previous tile
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
       }
     }
     __syncthreads();
-    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);  // in flight during the MFMAs
+    if (tile + (int)gridDim.x < ntiles) load_tile(tile + (int)gridDim.x);  // in flight during the MFMAs
     f32x4 acc[TR][NO];
 #pragma unroll
     for (int r = 0; r < TR; ++r)
@@ -390,11 +392,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
         }
     const int xx = tx0 + 16 * wave + lr;
     if (xx < a.W) {
+      uint16_t* dst0 = (uint16_t*)a.y + (((int64_t)img * a.H + ty0) * a.W + xx) * CO + 4 * lq;
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
         const int yy = ty0 + r;
         if (yy < a.H) {
-          uint16_t* dst = (uint16_t*)a.y + (((int64_t)img * a.H + yy) * a.W + xx) * CO + 4 * lq;
+          uint16_t* dst = dst0 + (int64_t)r * a.W * CO;
 #pragma unroll
           for (int n = 0; n < NO; ++n) {
             float v[4];
