@@ -79,6 +79,7 @@ _SIGS = {
     "apx_unet_train_step": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _int, _dbl, C.POINTER(_dbl), _vp]),
     "apx_plan_attach_unet": (_int, [_vp, _vp, _vp]),
     "apx_unet_set_jacobi_skip": (_int, [_vp, _int]),
+    "apx_unet_set_fused_tail": (_int, [_vp, _int, _vp]),
     "apx_fcg": (_int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
     "apx_set_timing": (_int, [_vp, _int]),
     "apx_get_timing": (_int, [_vp, C.c_char_p, C.POINTER(_dbl), C.POINTER(_i64), C.POINTER(_i64)]),

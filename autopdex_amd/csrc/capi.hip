@@ -429,6 +429,14 @@ int apx_unet_set_jacobi_skip(apx_unet* net, int enable) {
   });
 }
 
+int apx_unet_set_fused_tail(apx_unet* net, int enable, int* tail_level) {
+  return guarded([&] {
+    APX_REQUIRE(net, APX_ERR_ARG, "null network");
+    net->fuse_tail = enable ? 1 : 0;
+    if (tail_level) *tail_level = net->tail_level;
+  });
+}
+
 int apx_plan_attach_unet(apx_plan* plan, apx_unet* net, const double* g_nodal_dev) {
   return guarded([&] {
     Plan& P = ready(plan);

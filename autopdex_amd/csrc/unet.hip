@@ -146,19 +146,17 @@ __device__ __forceinline__ u32x4 bf8_pack(const float f[8]) {
   return q;
 }
 
-template <int C8>
-__global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
-                                                     uint16_t* out, int H, int W, int Hc, int Wc, const int* stop) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= W * C8 || (stop && *stop)) return;
-  const int j = idx / C8, c = (idx % C8) * 8, i = blockIdx.y, b = blockIdx.z;
-  constexpr int C = 8 * C8;
+// One 8-channel chunk of out = skip + P(coarse) at fine node (i, j); pointers at the image,
+// C channels per node.
+template <int C>
+__device__ __forceinline__ u32x4 prolong_chunk(const uint16_t* __restrict__ coarse, const uint16_t* skip, int i,
+                                               int j, int c, int W, int Hc, int Wc) {
   const int I0 = i >> 1, J0 = j >> 1;
   const int I1 = min((i & 1) ? I0 + 1 : I0, Hc - 1), J1 = min((j & 1) ? J0 + 1 : J0, Wc - 1);
-  const uint16_t* cb = coarse + (size_t)b * Hc * Wc * C + c;
+  const uint16_t* cb = coarse + c;
   float v[8], t[8];
   auto acc = [&](int I, int J, float w) {
-    bf8_unpack(*(const u32x4*)(cb + ((size_t)I * Wc + J) * C), t);
+    bf8_unpack(*(const u32x4*)(cb + (I * Wc + J) * C), t);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] += w * t[e];
   };
@@ -167,20 +165,64 @@ __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict_
   const bool oi = i & 1, oj = j & 1;
   const bool hi = oi && I0 + 1 < Hc, hj = oj && J0 + 1 < Wc;  // outside the coarse grid: zero
   const float wi0 = oi ? 0.5f : 1.f, wj0 = oj ? 0.5f : 1.f;
+  // every load is in bounds (I1, J1 clamped) and unconditional, so all four are in flight at
+  // once; a missing term adds w * 0 = +0, which leaves v unchanged (v starts at +0 and a sum
+  // of this form never becomes -0), so the bits match skipping it
+  auto acc_if = [&](bool on, int I, int J, float w) {
+    bf8_unpack(*(const u32x4*)(cb + (I * Wc + J) * C), t);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] += w * (on ? t[e] : 0.f);
+  };
   acc(I0, J0, wi0 * wj0);
-  if (hj) acc(I0, J1, wi0 * 0.5f);
-  if (hi) acc(I1, J0, 0.5f * wj0);
-  if (hi && hj) acc(I1, J1, 0.25f);
-  const size_t o = (((size_t)b * H + i) * W + j) * C + c;
+  acc_if(hj, I0, J1, wi0 * 0.5f);
+  acc_if(hi, I1, J0, 0.5f * wj0);
+  acc_if(hi && hj, I1, J1, 0.25f);
   if (skip) {
-    bf8_unpack(*(const u32x4*)(skip + o), t);
+    bf8_unpack(*(const u32x4*)(skip + (i * W + j) * C + c), t);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] += t[e];
   }
-  *(u32x4*)(out + o) = bf8_pack(v);
+  return bf8_pack(v);
 }
 
-// R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj).
+template <int C8>
+__global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
+                                                     uint16_t* out, int H, int W, int Hc, int Wc, const int* stop) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= W * C8 || (stop && *stop)) return;
+  const int j = idx / C8, c = (idx % C8) * 8, i = blockIdx.y, b = blockIdx.z;
+  constexpr int C = 8 * C8;
+  const size_t img = (size_t)H * W * C * b;
+  *(u32x4*)(out + img + ((size_t)i * W + j) * C + c) =
+      prolong_chunk<C>(coarse + (size_t)Hc * Wc * C * b, skip ? skip + img : nullptr, i, j, c, W, Hc, Wc);
+}
+
+// R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj); one 8-channel
+// chunk at coarse node (I, J), pointer at the image.
+template <int C>
+__device__ __forceinline__ u32x4 restrict_chunk(const uint16_t* __restrict__ fine, int I, int J, int c, int H, int W) {
+  const uint16_t* fb = fine + c;
+  float v[8], t[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = 0.f;
+  // nodes outside the fine grid: the load is clamped into the grid and its term is w * 0 = +0,
+  // which leaves v unchanged (v starts at +0 and never becomes -0), so the bits match skipping
+  // it; all nine loads are unconditional and in flight at once
+#pragma unroll
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int i = 2 * I + di, j = 2 * J + dj;
+      const int ic = min(max(i, 0), H - 1), jc = min(max(j, 0), W - 1);
+      const bool in = i == ic && j == jc;
+      const float w = (di ? 0.5f : 1.f) * (dj ? 0.5f : 1.f);
+      bf8_unpack(*(const u32x4*)(fb + (ic * W + jc) * C), t);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] += w * (in ? t[e] : 0.f);
+    }
+  return bf8_pack(v);
+}
+
 template <int C8>
 __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int H,
                                                   int W, int Hc, int Wc, const int* stop) {
@@ -188,22 +230,179 @@ __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ f
   if (idx >= Wc * C8 || (stop && *stop)) return;
   const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
-  const uint16_t* fb = fine + (size_t)b * H * W * C + c;
-  float v[8], t[8];
+  *(u32x4*)(out + (((size_t)b * Hc + I) * Wc + J) * C + c) =
+      restrict_chunk<C>(fine + (size_t)H * W * C * b, I, J, c, H, W);
+}
+
+// ---- coarse-level tail of the forward pass ----------------------------------------------
+// One workgroup per image runs the steps in order, with a block barrier after each (the
+// workgroup's global stores are visible to all its threads after the barrier).
+// Convolution steps form the same products in the same accumulation order as
+// k_conv3x3<64, 64, false, NB> (A = 16 pixels x 16 K, B = 16 K x 16 output channels, K
+// blocks tap-major, 16 channels each), with the same epilogue, so the results are
+// bit-identical to the per-layer launches.  Pixels are grouped 16 at a time in row-major
+// order over the whole grid; the input is staged in LDS with a zero halo ((H + 2) x (W + 2)
+// nodes, pixel stride kTailS), the weights as [co][k] rows of stride kTailSW.  Wavefront w
+// computes output channels 32 (w & 1) .. 32 (w & 1) + 31 of the pixel groups w / 2,
+// w / 2 + 4, ...
+// Weights arrive with 16-byte loads spread over the whole workgroup (a wavefront gathering
+// its own B fragments from global memory touches 16 cache lines per load, which on one
+// compute unit costs far more than the matrix work).  The next convolution's weights are
+// loaded into registers while the current step computes: they are issued after the step's
+// bias and input loads (loads complete in issue order as far as waits go), unconditionally
+// (the last convolution reloads its own), and the barrier between staging and compute waits
+// for LDS only.
+constexpr int kTailThreads = 512;
+constexpr int kTailS = 72;
+constexpr int kTailSW = 576 + 8;
+constexpr int kTailPix = (kTailMaxGrid + 2) * (kTailMaxGrid + 2);
+constexpr int kTailStage = (kTailPix * 8 + kTailThreads - 1) / kTailThreads;  // staged input chunks per thread
+constexpr int kTailWChunks = 64 * 576 / 8 / kTailThreads;                      // weight chunks per thread
+static_assert(64 * 576 / 8 % kTailThreads == 0, "weight chunks must tile the workgroup");
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int NM>
+__device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint16_t* sw, int m0, int H, int W, int cp,
+                                                 float bias0, float bias1, int relu, uint16_t* Y, int lr, int lg) {
+  const int WP = W + 2, HW = H * W;
+  int lp[NM];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) v[e] = 0.f;
+  for (int mm = 0; mm < NM; ++mm) {
+    const int p = min((m0 + 4 * mm) * 16 + lr, HW - 1);  // lanes past the grid repeat its last node
+    lp[mm] = ((p / W) * WP + p % W) * kTailS + 4 * lg;   // top-left node of the 3 x 3 window
+  }
+  const uint16_t* wb0 = sw + (32 * cp + lr) * kTailSW + 4 * lg;
+  const uint16_t* wb1 = wb0 + 16 * kTailSW;
+  f32x4 acc[NM][2];
 #pragma unroll
-  for (int di = -1; di <= 1; ++di)
+  for (int mm = 0; mm < NM; ++mm) acc[mm][0] = acc[mm][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int dj = -1; dj <= 1; ++dj) {
-      const int i = 2 * I + di, j = 2 * J + dj;
-      if (i < 0 || i >= H || j < 0 || j >= W) continue;
-      const float w = (di ? 0.5f : 1.f) * (dj ? 0.5f : 1.f);
-      bf8_unpack(*(const u32x4*)(fb + ((size_t)i * W + j) * C), t);
+  for (int kb = 0; kb < 36; ++kb) {
+    const int t = kb >> 2;  // tap 3 dy + dx, then 16-channel block kb & 3
+    const int off = ((t / 3) * WP + t % 3) * kTailS + (kb & 3) * 16;
+    const bf16x4 b0 = *(const bf16x4*)(wb0 + kb * 16), b1 = *(const bf16x4*)(wb1 + kb * 16);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] += w * t[e];
+    for (int mm = 0; mm < NM; ++mm) {
+      const bf16x4 af = *(const bf16x4*)(sx + lp[mm] + off);
+      acc[mm][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, b0, acc[mm][0], 0, 0, 0);
+      acc[mm][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, b1, acc[mm][1], 0, 0, 0);
     }
-  *(u32x4*)(out + (((size_t)b * Hc + I) * Wc + J) * C + c) = bf8_pack(v);
+  }
+#pragma unroll
+  for (int n = 0; n < 2; ++n) {
+    const int co = 32 * cp + 16 * n + lr;
+    const float bb = n ? bias1 : bias0;
+#pragma unroll
+    for (int mm = 0; mm < NM; ++mm)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = (m0 + 4 * mm) * 16 + 4 * lg + j;
+        float v = acc[mm][n][j] + bb;
+        if (relu) v = fmaxf(v, 0.f);
+        if (p < HW) Y[p * 64 + co] = f2bf(v);
+      }
+  }
+}
+
+__global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const int* stop) {
+  if (stop && *stop) return;  // uniform over the grid: set before the launch
+  __shared__ __attribute__((aligned(16))) uint16_t sx[kTailPix * kTailS];
+  __shared__ __attribute__((aligned(16))) uint16_t sw[64 * kTailSW];
+  const int img = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int cp = wave & 1, ph = wave >> 1;
+  u32x4 qw[kTailWChunks];
+  auto fetch_weights = [&](const uint16_t* w) {
+#pragma unroll
+    for (int r = 0; r < kTailWChunks; ++r) {
+      const int i = threadIdx.x + r * kTailThreads;
+      qw[r] = *(const u32x4*)(w + (i / 72) * 576 + (i % 72) * 8);
+    }
+  };
+  fetch_weights(args.w_first);
+  for (int si = 0; si < args.n; ++si) {
+    const TailStep& st = args.step[si];
+    if (st.op == kTailConv) {
+      const int H = st.H, W = st.W, WP = W + 2, NP8 = (H + 2) * WP * 8;
+      const int64_t io = (int64_t)img * H * W * 64;
+      const uint16_t* X = st.x + io;
+      // the prefetched weights -> LDS (every reader of the previous step passed its barrier)
+#pragma unroll
+      for (int r = 0; r < kTailWChunks; ++r) {
+        const int i = threadIdx.x + r * kTailThreads;
+        *(u32x4*)(sw + (i / 72) * kTailSW + (i % 72) * 8) = qw[r];
+      }
+      const float bias0 = st.bias[32 * cp + lr], bias1 = st.bias[32 * cp + 16 + lr];
+      u32x4 q[kTailStage];
+#pragma unroll
+      for (int r = 0; r < kTailStage; ++r) {
+        const int i = threadIdx.x + r * kTailThreads;
+        const int pp = i >> 3, py = pp / WP - 1, px = pp % WP - 1;
+        q[r] = u32x4{0u, 0u, 0u, 0u};
+        if (i < NP8 && py >= 0 && py < H && px >= 0 && px < W)
+          q[r] = *(const u32x4*)(X + (py * W + px) * 64 + (i & 7) * 8);
+      }
+      fetch_weights(st.w_next);  // in flight through this step's matrix work
+#pragma unroll
+      for (int r = 0; r < kTailStage; ++r) {
+        const int i = threadIdx.x + r * kTailThreads;
+        if (i < NP8) *(u32x4*)(sx + (i >> 3) * kTailS + (i & 7) * 8) = q[r];
+      }
+      lds_barrier();
+      uint16_t* Y = st.y + io;
+      const int NG = (H * W + 15) / 16;
+      for (int m0 = ph; m0 < NG; m0 += 16) {
+        switch (min(4, (NG - m0 + 3) / 4)) {
+          case 4: tail_conv_groups<4>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
+          case 3: tail_conv_groups<3>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
+          case 2: tail_conv_groups<2>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
+          default: tail_conv_groups<1>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
+        }
+      }
+    } else if (st.op == kTailRestrict) {
+      const int n8 = st.Hc * st.Wc * 8;
+      const uint16_t* F = st.x + (int64_t)img * st.H * st.W * 64;
+      uint16_t* Y = st.y + (int64_t)img * st.Hc * st.Wc * 64;
+      for (int i0 = threadIdx.x; i0 < n8; i0 += 4 * kTailThreads) {
+        u32x4 r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u * kTailThreads, node = i >> 3;
+          r[u] = i < n8 ? restrict_chunk<64>(F, node / st.Wc, node % st.Wc, (i & 7) * 8, st.H, st.W)
+                        : u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u * kTailThreads;
+          if (i < n8) *(u32x4*)(Y + (i >> 3) * 64 + (i & 7) * 8) = r[u];
+        }
+      }
+    } else {
+      const int n8 = st.H * st.W * 8;
+      const int64_t io = (int64_t)img * st.H * st.W * 64;
+      const uint16_t* Cg = st.x + (int64_t)img * st.Hc * st.Wc * 64;
+      for (int i0 = threadIdx.x; i0 < n8; i0 += 4 * kTailThreads) {
+        u32x4 r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u * kTailThreads, node = i >> 3;
+          r[u] = i < n8 ? prolong_chunk<64>(Cg, st.skip + io, node / st.W, node % st.W, (i & 7) * 8, st.W, st.Hc, st.Wc)
+                        : u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u * kTailThreads;
+          if (i < n8) *(u32x4*)(st.y + io + (i >> 3) * 64 + (i & 7) * 8) = r[u];
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---- parameters ---------------------------------------------------------------------------
@@ -397,6 +596,69 @@ void UNet::build(int L, const int* chans, int cin, int cout, int act, int B, int
   loss_part.alloc((int64_t)batch * kLossBlocks * 2 + 2 * batch + 1);
   scale.alloc(batch);
   weights_ready = false;
+  build_tail();
+}
+
+// The tail starts at the finest level c whose restriction begins a run of 64-channel levels
+// with convolution grids of at most kTailMaxGrid nodes a side (and a step list that fits).
+void UNet::build_tail() {
+  const int L = levels;
+  tail = TailArgs{};
+  tail_level = -1;
+  for (int c = 1; c + 1 < L && tail_level < 0; ++c) {
+    bool ok = H[c + 1] <= kTailMaxGrid && W[c + 1] <= kTailMaxGrid && 6 * (L - 1 - c) - 1 <= kTailMaxSteps;
+    for (int l = c; l < L; ++l) ok = ok && ch[l] == 64;
+    if (ok) tail_level = c;
+  }
+  if (tail_level < 0) return;
+  const int c = tail_level;
+  auto conv = [&](int li, int level, const uint16_t* x, uint16_t* y) {
+    const NetLayer& ly = layers[li];
+    TailStep& ts = tail.step[tail.n++];
+    ts.op = kTailConv;
+    ts.x = x;
+    ts.y = y;
+    ts.w = wf.p + ly.wf_off;
+    ts.bias = theta.p + ly.b_off;
+    ts.relu = ly.relu;
+    ts.H = H[level];
+    ts.W = W[level];
+  };
+  auto transfer = [&](int op, int l, const uint16_t* x, const uint16_t* skip, uint16_t* y) {
+    TailStep& ts = tail.step[tail.n++];
+    ts.op = op;
+    ts.x = x;
+    ts.skip = skip;
+    ts.y = y;
+    ts.H = H[l];
+    ts.W = W[l];
+    ts.Hc = H[l + 1];
+    ts.Wc = W[l + 1];
+  };
+  transfer(kTailRestrict, c, d[c].p, nullptr, rd[c].p);
+  conv(L_down[c], c + 1, rd[c].p, a[c + 1].p);
+  for (int l = c + 1; l + 1 < L; ++l) {
+    conv(L_e[l], l, a[l].p, d[l].p);
+    transfer(kTailRestrict, l, d[l].p, nullptr, rd[l].p);
+    conv(L_down[l], l + 1, rd[l].p, a[l + 1].p);
+  }
+  conv(L_b, L - 1, a[L - 1].p, u[L - 1].p);
+  for (int l = L - 2; l > c; --l) {
+    conv(L_up[l], l + 1, u[l + 1].p, t[l].p);
+    transfer(kTailProlong, l, t[l].p, d[l].p, s[l].p);
+    conv(L_d[l], l, s[l].p, u[l].p);
+  }
+  conv(L_up[c], c + 1, u[c + 1].p, t[c].p);
+  transfer(kTailProlong, c, t[c].p, d[c].p, s[c].p);
+  APX_REQUIRE(tail.n == 6 * (L - 1 - c) - 1, APX_ERR_STATE, "coarse-level tail: step count");
+  // each convolution prefetches the weights of the next one (the last one its own)
+  const uint16_t* next = nullptr;
+  for (int i = tail.n - 1; i >= 0; --i)
+    if (tail.step[i].op == kTailConv) {
+      tail.step[i].w_next = next ? next : tail.step[i].w;
+      next = tail.step[i].w;
+    }
+  tail.w_first = next;
 }
 
 void UNet::prepare_weights(hipStream_t st) {
@@ -469,7 +731,8 @@ void conv_wgrad(UNet& N, int li, int level, const uint16_t* x, const uint16_t* d
 void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const uint16_t* skip, uint16_t* out,
                  hipStream_t st) {
   const int H = N.H[level_fine], W = N.W[level_fine];
-  APX_REQUIRE(C % 8 == 0 && H < 65536 && N.batch < 65536, APX_ERR_UNSUPPORTED, "grid transfer shape");
+  APX_REQUIRE(C % 8 == 0 && H < 65536 && N.batch < 65536 && (int64_t)H * W * C < INT32_MAX, APX_ERR_UNSUPPORTED,
+              "grid transfer shape");
   const dim3 grid((W * (C / 8) + 255) / 256, H, N.batch);
   const int Hc = N.H[level_fine + 1], Wc = N.W[level_fine + 1];
   if (C == 16)
@@ -485,7 +748,8 @@ void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const u
 void restrict_(UNet& N, int level_fine, int C, const uint16_t* fine, uint16_t* out, hipStream_t st) {
   const int H = N.H[level_fine], W = N.W[level_fine];
   const int Hc = N.H[level_fine + 1], Wc = N.W[level_fine + 1];
-  APX_REQUIRE(C % 8 == 0 && Hc < 65536 && N.batch < 65536, APX_ERR_UNSUPPORTED, "grid transfer shape");
+  APX_REQUIRE(C % 8 == 0 && Hc < 65536 && N.batch < 65536 && (int64_t)H * W * C < INT32_MAX, APX_ERR_UNSUPPORTED,
+              "grid transfer shape");
   const dim3 grid((Wc * (C / 8) + 255) / 256, Hc, N.batch);
   if (C == 16)
     hipLaunchKernelGGL(k_restrict<2>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
@@ -525,8 +789,11 @@ void UNet::forward(hipStream_t st) {
   const int L = levels;
   const bool timed = timer_plan && timer_plan->timing;
   EventPair ef = timed ? timing_begin(*timer_plan, st) : EventPair{};
+  // levels >= c from the restriction of level c to its prolongation: one launch (the tail)
+  const bool fused = fuse_tail && tail_level >= 0;
+  const int c = fused ? tail_level : L;
   conv_fwd(*this, L_in, 0, x0.p, a[0].p, st);
-  for (int l = 0; l + 1 < L; ++l) {
+  for (int l = 0; l + 1 < L && l <= c; ++l) {
     EventPair ec{};
     if (timed && l == 0) ec = timing_begin(*timer_plan, st);
     conv_fwd(*this, L_e[l], l, a[l].p, d[l].p, st);
@@ -534,13 +801,19 @@ void UNet::forward(hipStream_t st) {
       timing_end(*timer_plan, ec, st);
       ev_conv0.push_back(ec);
     }
+    if (l == c) break;
     restrict_(*this, l, ch[l], d[l].p, rd[l].p, st);
     conv_fwd(*this, L_down[l], l + 1, rd[l].p, a[l + 1].p, st);
   }
-  conv_fwd(*this, L_b, L - 1, a[L - 1].p, u[L - 1].p, st);
-  for (int l = L - 2; l >= 0; --l) {
-    conv_fwd(*this, L_up[l], l + 1, u[l + 1].p, t[l].p, st);
-    prolong_add(*this, l, ch[l], t[l].p, d[l].p, s[l].p, st);
+  if (fused)
+    hipLaunchKernelGGL(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop);
+  else
+    conv_fwd(*this, L_b, L - 1, a[L - 1].p, u[L - 1].p, st);
+  for (int l = std::min(L - 2, c); l >= 0; --l) {
+    if (l != c) {
+      conv_fwd(*this, L_up[l], l + 1, u[l + 1].p, t[l].p, st);
+      prolong_add(*this, l, ch[l], t[l].p, d[l].p, s[l].p, st);
+    }
     conv_fwd(*this, L_d[l], l, s[l].p, u[l].p, st);
   }
   conv_fwd(*this, L_out, 0, u[0].p, y.p, st);

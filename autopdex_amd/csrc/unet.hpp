@@ -17,6 +17,32 @@ struct NetLayer {
   int d_cop = 0, d_wk = 0;       // dgrad conv: ci' = co, co' = ci
 };
 
+// The coarse levels of the forward pass run as one launch, one workgroup per image: a list of
+// 64 -> 64 channel convolutions and grid transfers (device pointers at image 0), each step
+// reading what the previous ones wrote.  Convolution grids stay within 9 x 9 nodes: a step's
+// padded input and weights fit in LDS, and the matrix work of one step is small enough for
+// one compute unit (at 17 x 17 nodes the per-layer launches are faster).
+enum TailOp : int { kTailConv = 0, kTailRestrict = 1, kTailProlong = 2 };
+struct TailStep {
+  const uint16_t* x;     // convolution input; restriction: fine grid; prolongation: coarse grid
+  const uint16_t* skip;  // prolongation: the skip tensor added to P(x)
+  uint16_t* y;           // output
+  const uint16_t* w;     // convolution weights [64][576] (bf16, forward layout)
+  const uint16_t* w_next;  // convolution: weights of the next convolution step (prefetched; the last: w)
+  const float* bias;     // convolution bias (64)
+  int op;                // TailOp
+  int relu;
+  int H, W;              // convolution grid; transfers: the fine grid
+  int Hc, Wc;            // transfers: the coarse grid
+};
+constexpr int kTailMaxSteps = 28;  // 2 KB of kernel arguments
+constexpr int kTailMaxGrid = 9;    // largest convolution grid side in the tail
+struct TailArgs {
+  TailStep step[kTailMaxSteps];
+  const uint16_t* w_first;  // weights of the first convolution step
+  int n;
+};
+
 struct UNet {
   int levels = 0, c_in = 0, c_out = 0, relu = 0, batch = 1;
   int ch[kMaxLevels] = {};
@@ -46,8 +72,14 @@ struct UNet {
   const int* stop = nullptr;
   // 1: the network is a correction to Jacobi, z = D^-1 r + D^-1/2 s net(D^-1/2 r / s)
   int jacobi_skip = 0;
+  // coarse-level tail: levels >= tail_level (from its restriction to its prolongation) run as
+  // one launch when fuse_tail is set; tail_level = -1: no such levels
+  int tail_level = -1;
+  int fuse_tail = 1;
+  TailArgs tail{};
 
   void build(int levels, const int* channels, int c_in, int c_out, int relu, int batch, int H0, int W0);
+  void build_tail();                      // tail_level and the tail's step list (after build's allocations)
   void prepare_weights(hipStream_t st);   // FP32 master -> bf16 forward / backward-data layouts
   void forward(hipStream_t st);           // x0 -> y
   // loss = mean_b ||y_b - t_b||^2 / ||t_b||^2 ; fills grad (FP32) via backward

@@ -107,6 +107,13 @@ class UNetPreconditioner:
         _call("apx_unet_buffers", self.handle, C.byref(x), C.byref(y), C.byref(s))
         return x.value, y.value, s.value
 
+    def set_fused_tail(self, enable):
+        """Coarse levels in one launch (True, the default) or one launch per layer; returns the
+        level whose restriction starts the fused part (-1: none)."""
+        level = C.c_int()
+        _call("apx_unet_set_fused_tail", self.handle, int(bool(enable)), C.byref(level))
+        return level.value
+
     def forward(self):
         _call("apx_unet_forward", self.handle, current_stream())
 

@@ -4,6 +4,8 @@
   (bf16 activations emulated in the oracle; tolerance: relative L2 error 2e-2
   for outputs, 5e-2 per layer for gradients -- accumulation order and bf16
   rounding-boundary effects only);
+- the coarse levels fused into one launch give the same bits as one launch per
+  layer (outputs and gradients), and match the oracle;
 - a network trained on the GPU on pairs from the plain solver needs fewer
   Krylov iterations than Jacobi-PCG and reaches the SuperLU solution to 1e-8;
 - a broken network triggers the Jacobi-PCG fallback and Newton still converges;
@@ -75,6 +77,75 @@ def test_forward_and_gradients_match_oracle(torch_gpu, relu, c_in, c_out):
         gw, gw_ref = g[w_off:b_off + co], g_ref[w_off:b_off + co]
         err = np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30)
         assert err < 5e-2, (name, err)
+
+
+def _tail_level(H, W, L, ch):
+    """First level c >= 1 whose restriction starts a run of 64-channel levels with grids of at
+    most 9 x 9 nodes from level c + 1 on (the fused coarse-level launch)."""
+    Hs, Ws = [H], [W]
+    for _ in range(1, L):
+        Hs.append((Hs[-1] + 1) // 2)
+        Ws.append((Ws[-1] + 1) // 2)
+    for c in range(1, L - 1):
+        if Hs[c + 1] <= 9 and Ws[c + 1] <= 9 and all(ch[l] == 64 for l in range(c, L)) and 6 * (L - 1 - c) - 1 <= 28:
+            return c
+    return -1
+
+
+TAIL_CASES = [
+    # H, W, levels, channels, batch, relu, oracle check
+    (33, 20, 5, (16, 32, 64, 64, 64), 2, 1, True),           # tail from level 2: 9 x 5 ... 3 x 2
+    (129, 97, 7, (16, 32) + (64,) * 5, 2, 0, False),         # tail from level 3: 9 x 7 ... 3 x 2
+    (2049, 2049, 11, (16, 32) + (64,) * 9, 1, 0, False),     # Poisson 2048 x 2048 network, tail from level 7
+]
+
+
+@pytest.mark.parametrize("H,W,L,ch,B,relu,oracle", TAIL_CASES)
+def test_fused_coarse_tail_is_bit_identical_to_per_layer_launches(torch_gpu, H, W, L, ch, B, relu, oracle):
+    """The coarse levels in one launch (one workgroup per image) give the same bits as one
+    launch per layer, for the output and the parameter gradients; on the small case both
+    also match the numpy oracle (tolerances as above)."""
+    torch = torch_gpu
+    from autopdex_amd.learned import UNetPreconditioner
+    print(f"[case] fused tail H={H} W={W} L={L} B={B} relu={relu}", flush=True)
+    net = UNetPreconditioner(H, W, L, ch, 1, 1, relu=bool(relu), batch=B, seed=7)
+    rng = np.random.default_rng(3)
+    theta = net.get_params()
+    for ly in net.layers:
+        theta[ly["b_off"]:ly["b_off"] + ly["co"]] = 0.05 * rng.standard_normal(ly["co"])
+    net.set_params(theta)
+    level = net.set_fused_tail(True)
+    assert level == _tail_level(H, W, L, ch) and level >= 0
+    x = rng.standard_normal((B, H, W, 1)).astype(np.float32)
+    xb = torch.from_numpy(_bf16_bits(x).view(np.int16)).cuda()
+    xp, yp, sp = net.buffers()
+    _copy(xp, xb.data_ptr(), xb.numel() * 2)
+    t = torch.from_numpy(rng.standard_normal(B * H * W).astype(np.float32)).cuda()
+    outs = {}
+    for fused in (True, False):
+        net.set_fused_tail(fused)
+        net.forward()
+        y = torch.empty(B * H * W, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        _copy(y.data_ptr(), yp, y.numel() * 4)
+        loss = net.loss_backward(t)
+        outs[fused] = (y.cpu().numpy(), loss, net.get_grads())
+    (y1, l1, g1), (y2, l2, g2) = outs[True], outs[False]
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+    assert l1 == l2
+    assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32))
+    assert np.isfinite(y1).all() and np.abs(y1).max() > 0
+    if oracle:
+        orc = OU.UNetOracle(L, ch, 1, 1, bool(relu), H, W, bf16_storage=True)
+        y_ref = orc.forward(theta.astype(np.float64), OU.bf16(x))
+        y1 = y1.reshape(B, H, W, 1)
+        assert np.linalg.norm(y1 - y_ref) / np.linalg.norm(y_ref) < 2e-2
+        loss_ref, g_ref = orc.loss_backward(t.cpu().numpy().astype(np.float64).reshape(B, H, W, 1))
+        assert abs(l1 - loss_ref) / loss_ref < 2e-2
+        for name, (w_off, b_off, ci, co, cop, wk) in orc.layout.items():
+            gw, gw_ref = g1[w_off:b_off + co], g_ref[w_off:b_off + co]
+            err = np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30)
+            assert err < 5e-2, (name, err)
 
 
 def _poisson_problem(n):
