@@ -189,12 +189,13 @@ template <int C8>
 __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
                                                      uint16_t* out, int H, int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= W * C8 || (stop && *stop)) return;
+  if (idx >= W * C8) return;
   const int j = idx / C8, c = (idx % C8) * 8, i = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
   const size_t img = (size_t)H * W * C * b;
-  *(u32x4*)(out + img + ((size_t)i * W + j) * C + c) =
-      prolong_chunk<C>(coarse + (size_t)Hc * Wc * C * b, skip ? skip + img : nullptr, i, j, c, W, Hc, Wc);
+  const u32x4 v = prolong_chunk<C>(coarse + (size_t)Hc * Wc * C * b, skip ? skip + img : nullptr, i, j, c, W, Hc, Wc);
+  if (stop && *stop) return;  // tested after the loads are issued: the round trips overlap
+  *(u32x4*)(out + img + ((size_t)i * W + j) * C + c) = v;
 }
 
 // R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj); one 8-channel
@@ -227,11 +228,12 @@ template <int C8>
 __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int H,
                                                   int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= Wc * C8 || (stop && *stop)) return;
+  if (idx >= Wc * C8) return;
   const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
-  *(u32x4*)(out + (((size_t)b * Hc + I) * Wc + J) * C + c) =
-      restrict_chunk<C>(fine + (size_t)H * W * C * b, I, J, c, H, W);
+  const u32x4 v = restrict_chunk<C>(fine + (size_t)H * W * C * b, I, J, c, H, W);
+  if (stop && *stop) return;  // tested after the loads are issued: the round trips overlap
+  *(u32x4*)(out + (((size_t)b * Hc + I) * Wc + J) * C + c) = v;
 }
 
 // ---- coarse-level tail of the forward pass ----------------------------------------------
@@ -310,7 +312,6 @@ __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint1
 }
 
 __global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const int* stop) {
-  if (stop && *stop) return;  // uniform over the grid: set before the launch
   __shared__ __attribute__((aligned(16))) uint16_t sx[kTailPix * kTailS];
   __shared__ __attribute__((aligned(16))) uint16_t sw[64 * kTailSW];
   const int img = blockIdx.x;
@@ -325,6 +326,9 @@ __global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const
     }
   };
   fetch_weights(args.w_first);
+  // stop flag: uniform over the grid (set before the launch); read after the first loads are
+  // issued, so that its round trip overlaps theirs
+  if (stop && *stop) return;
   for (int si = 0; si < args.n; ++si) {
     const TailStep& st = args.step[si];
     if (st.op == kTailConv) {
@@ -915,8 +919,8 @@ template <class T>
 __global__ void k_to_grid(int64_t n_nodes, int d, int cin, const uint8_t* dmask, const int32_t* full2red, const T* r,
                           const double* dsq, const double* g, const float* scale, int img, uint16_t* x,
                           const int* stop) {
+  const float inv = 1.f / scale[img];  // issued together with the stop flag's load
   if (stop && *stop) return;
-  const float inv = 1.f / scale[img];
   for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < n_nodes;
        node += (int64_t)gridDim.x * blockDim.x) {
     uint16_t* xo = x + ((int64_t)img * n_nodes + node) * cin;

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -95,69 +96,85 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   static_assert(OUT_F32 || CB % 8 == 0, "bf16 outputs are stored in 8-channel chunks");
   __shared__ __attribute__((aligned(16))) uint16_t sbuf[SBUF];
   __shared__ __attribute__((aligned(16))) uint16_t sw[16 * NB * SWK];
-  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   uint16_t* sx = sbuf;
   const int co0 = blockIdx.y * 16 * NB;  // first output channel of this block
   // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; the
   // weight loads and the first tile's loads are all issued before the weights are stored, so
   // that the two round trips overlap (it matters on small grids, where a launch has one tile)
   constexpr int NW = 16 * NB * (WK / 8), RW = (NW + kConvThreads - 1) / kConvThreads;
+  // every load below is unconditional, at an address clamped into its array, so that all of a
+  // thread's loads are in flight at once (a load under a per-lane condition makes the compiler
+  // wait for it at the branch join); what lies outside is dropped or zeroed at the LDS store
   u32x4 qw[RW];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
-    const int i = threadIdx.x + r * kConvThreads;
-    if (i < NW) qw[r] = *(const u32x4*)(a.w + (int64_t)(co0 + i / (WK / 8)) * WK + (i % (WK / 8)) * 8);
+    const int i = min((int)threadIdx.x + r * kConvThreads, NW - 1);
+    qw[r] = *(const u32x4*)(a.w + (int64_t)(co0 + i / (WK / 8)) * WK + (i % (WK / 8)) * 8);
   }
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + kTileH - 1) / kTileH;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lg = lane >> 4;
-  // input tile loads with a one-pixel halo (zero outside the image) into registers
+  // input tile loads with a one-pixel halo into registers; bit r of qin: chunk r lies inside
+  // the image (otherwise it is stored as zeros)
+  static_assert(RC <= 32, "one validity bit per staged chunk");
   u32x4 qs[RC];
-  auto load_tile = [&](int64_t tile) {
+  uint32_t qin = 0;
+  auto load_tile = [&](int64_t tile, auto masked) __attribute__((always_inline)) {
+    constexpr bool M = decltype(masked)::value;  // ReLU backward: X *= (xmask > 0)
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
     const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;
     const int64_t img_off = (int64_t)img * a.H * a.W * CI;
     const uint16_t* xb = a.x + img_off;
+    qin = 0;
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
       const int i = threadIdx.x + r * kConvThreads;
       const int pix = i / CH8, ch = (i % CH8) * 8;
       const int yy = ty0 - 1 + pix / (kTileW + 2), xx = tx0 - 1 + pix % (kTileW + 2);
-      const bool in = i < NC && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-      u32x4 q = u32x4{0u, 0u, 0u, 0u};
-      if (in) {
-        const int64_t o = ((int64_t)yy * a.W + xx) * CI + ch;
-        if constexpr (CI % 8 == 0) {
-          q = *(const u32x4*)(xb + o);
-          if (a.xmask) {
-            const u32x4 m = *(const u32x4*)(a.xmask + img_off + o);
+      qin |= (uint32_t)(i < NC && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) << r;
+      const int yc = min(max(yy, 0), a.H - 1), xc = min(max(xx, 0), a.W - 1);
+      const int64_t o = ((int64_t)yc * a.W + xc) * CI + ch;
+      u32x4 q;
+      if constexpr (CI % 8 == 0) {
+        q = *(const u32x4*)(xb + o);
+        if constexpr (M) {
+          const u32x4 m = *(const u32x4*)(a.xmask + img_off + o);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              uint32_t keep = 0u;
-              if (bf2f((uint16_t)(m[e] & 0xffffu)) > 0.f) keep |= 0xffffu;
-              if (bf2f((uint16_t)(m[e] >> 16)) > 0.f) keep |= 0xffff0000u;
-              q[e] &= keep;
-            }
+          for (int e = 0; e < 4; ++e) {
+            uint32_t keep = 0u;
+            if (bf2f((uint16_t)(m[e] & 0xffffu)) > 0.f) keep |= 0xffffu;
+            if (bf2f((uint16_t)(m[e] >> 16)) > 0.f) keep |= 0xffff0000u;
+            q[e] &= keep;
           }
-        } else {
-          uint16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            if (ch + e < CI) {
-              v[e] = xb[o + e];
-              if (a.xmask && !(bf2f(a.xmask[img_off + o + e]) > 0.f)) v[e] = 0;
-            }
-#pragma unroll
-          for (int e = 0; e < 4; ++e) q[e] = (uint32_t)v[2 * e] | ((uint32_t)v[2 * e + 1] << 16);
         }
+      } else {
+        uint16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (ch + e < CI) {
+            v[e] = xb[o + e];
+            if constexpr (M)
+              if (!(bf2f(a.xmask[img_off + o + e]) > 0.f)) v[e] = 0;
+          }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[e] = (uint32_t)v[2 * e] | ((uint32_t)v[2 * e + 1] << 16);
       }
       qs[r] = q;
     }
   };
+  auto load_tile_any = [&](int64_t tile) __attribute__((always_inline)) {
+    if (a.xmask)
+      load_tile(tile, std::true_type{});
+    else
+      load_tile(tile, std::false_type{});
+  };
   const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
-  if (tile0 < ntiles) load_tile(tile0);
+  if (tile0 < ntiles) load_tile_any(tile0);
+  // stop flag: uniform over the grid (set before the launch); read after the first loads are
+  // issued, so that its round trip overlaps theirs
+  if (a.stop && *a.stop) return;
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int i = threadIdx.x + r * kConvThreads;
@@ -171,10 +188,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
       const int i = threadIdx.x + r * kConvThreads;
-      if (i < NC) *(u32x4*)(sx + (i / CH8) * S + (i % CH8) * 8) = qs[r];
+      if (i < NC) *(u32x4*)(sx + (i / CH8) * S + (i % CH8) * 8) = ((qin >> r) & 1u) ? qs[r] : u32x4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);  // in flight during the MFMAs
+    if (tile + gridDim.x < ntiles) load_tile_any(tile + gridDim.x);  // in flight during the MFMAs
     // each wavefront: one tile row, 4 row tiles of 16 pixels, NB column tiles
     f32x4 acc[4][NB];
 #pragma unroll
@@ -289,7 +306,6 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
   constexpr int RC = (NL + kConvThreads - 1) / kConvThreads;
   constexpr int WK = 9 * CI;                 // weight row length
   __shared__ __attribute__((aligned(16))) uint16_t sx[(CI / 4) * PP * 4];
-  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lq = lane >> 4;
   bf16x4 wa[9][NC][NO];
@@ -354,6 +370,9 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
   };
   const int tile0 = (int)xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
   if (tile0 < ntiles) load_tile(tile0);
+  // stop flag: uniform over the grid (set before the launch); read after the first loads are
+  // issued, so that its round trip overlaps theirs
+  if (a.stop && *a.stop) return;
   for (int tile = tile0; tile < ntiles; tile += gridDim.x) {
     const int img = tile / tiles_img, trem = tile - img * tiles_img;
     const int tyi = trem / tiles_x;
