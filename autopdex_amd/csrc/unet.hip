@@ -185,17 +185,79 @@ __device__ __forceinline__ u32x4 prolong_chunk(const uint16_t* __restrict__ coar
   return bf8_pack(v);
 }
 
+// out = skip + P(coarse), one thread per coarse node and 8-channel chunk, writing the (up to)
+// four fine nodes (2I + di, 2J + dj): each coarse chunk is loaded and unpacked once.  The sums
+// are those of prolong_chunk with its zero terms dropped (adding w * 0 = +0 leaves a sum that
+// is never -0 unchanged), in the same order, so the bits agree with it.
 template <int C8>
 __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
                                                      uint16_t* out, int H, int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= W * C8) return;
-  const int j = idx / C8, c = (idx % C8) * 8, i = blockIdx.y, b = blockIdx.z;
+  if (idx >= Wc * C8) return;
+  const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
-  const size_t img = (size_t)H * W * C * b;
-  const u32x4 v = prolong_chunk<C>(coarse + (size_t)Hc * Wc * C * b, skip ? skip + img : nullptr, i, j, c, W, Hc, Wc);
+  const uint16_t* cb = coarse + (size_t)Hc * Wc * C * b + c;
+  const size_t fb = (size_t)H * W * C * b + c;
+  const bool hi = I + 1 < Hc, hj = J + 1 < Wc;      // the next coarse row / column exists
+  const bool ri = 2 * I + 1 < H, rj = 2 * J + 1 < W;  // the odd fine row / column exists
+  const int I1 = hi ? I + 1 : I, J1 = hj ? J + 1 : J;
+  const int i0 = 2 * I, j0 = 2 * J, i1 = ri ? i0 + 1 : i0, j1 = rj ? j0 + 1 : j0;
+  // every load unconditional and in bounds, all in flight at once
+  const u32x4 q00 = *(const u32x4*)(cb + (I * Wc + J) * C), q01 = *(const u32x4*)(cb + (I * Wc + J1) * C);
+  const u32x4 q10 = *(const u32x4*)(cb + (I1 * Wc + J) * C), q11 = *(const u32x4*)(cb + (I1 * Wc + J1) * C);
+  const size_t o00 = fb + ((size_t)i0 * W + j0) * C, o01 = fb + ((size_t)i0 * W + j1) * C;
+  const size_t o10 = fb + ((size_t)i1 * W + j0) * C, o11 = fb + ((size_t)i1 * W + j1) * C;
+  u32x4 s00{0u, 0u, 0u, 0u}, s01 = s00, s10 = s00, s11 = s00;
+  if (skip) {
+    s00 = *(const u32x4*)(skip + o00);
+    s01 = *(const u32x4*)(skip + o01);
+    s10 = *(const u32x4*)(skip + o10);
+    s11 = *(const u32x4*)(skip + o11);
+  }
   if (stop && *stop) return;  // tested after the loads are issued: the round trips overlap
-  *(u32x4*)(out + img + ((size_t)i * W + j) * C + c) = v;
+  float t00[8], t01[8], t10[8], t11[8], sk[8], v[8];
+  bf8_unpack(q00, t00);
+  bf8_unpack(q01, t01);
+  bf8_unpack(q10, t10);
+  bf8_unpack(q11, t11);
+  auto finish = [&](const u32x4& sq, size_t o) {
+    if (skip) {
+      bf8_unpack(sq, sk);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] += sk[e];
+    }
+    *(u32x4*)(out + o) = bf8_pack(v);
+  };
+  // (2I, 2J): weight 1 on (I, J)
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = 0.f + t00[e];
+  finish(s00, o00);
+  if (rj) {  // (2I, 2J + 1): 1/2 on (I, J) and (I, J + 1)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = 0.f + 0.5f * t00[e];
+      if (hj) v[e] += 0.5f * t01[e];
+    }
+    finish(s01, o01);
+  }
+  if (ri) {  // (2I + 1, 2J): 1/2 on (I, J) and (I + 1, J)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = 0.f + 0.5f * t00[e];
+      if (hi) v[e] += 0.5f * t10[e];
+    }
+    finish(s10, o10);
+  }
+  if (ri && rj) {  // (2I + 1, 2J + 1): 1/4 on the four
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = 0.f + 0.25f * t00[e];
+      if (hj) v[e] += 0.25f * t01[e];
+      if (hi) v[e] += 0.25f * t10[e];
+      if (hi && hj) v[e] += 0.25f * t11[e];
+    }
+    finish(s11, o11);
+  }
 }
 
 // R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj); one 8-channel
@@ -735,10 +797,10 @@ void conv_wgrad(UNet& N, int li, int level, const uint16_t* x, const uint16_t* d
 void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const uint16_t* skip, uint16_t* out,
                  hipStream_t st) {
   const int H = N.H[level_fine], W = N.W[level_fine];
-  APX_REQUIRE(C % 8 == 0 && H < 65536 && N.batch < 65536 && (int64_t)H * W * C < INT32_MAX, APX_ERR_UNSUPPORTED,
-              "grid transfer shape");
-  const dim3 grid((W * (C / 8) + 255) / 256, H, N.batch);
   const int Hc = N.H[level_fine + 1], Wc = N.W[level_fine + 1];
+  APX_REQUIRE(C % 8 == 0 && Hc < 65536 && N.batch < 65536 && (int64_t)H * W * C < INT32_MAX, APX_ERR_UNSUPPORTED,
+              "grid transfer shape");
+  const dim3 grid((Wc * (C / 8) + 255) / 256, Hc, N.batch);  // one thread per coarse node and chunk
   if (C == 16)
     hipLaunchKernelGGL(k_prolong_add<2>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else if (C == 32)

@@ -5,7 +5,7 @@
   for outputs, 5e-2 per layer for gradients -- accumulation order and bf16
   rounding-boundary effects only);
 - the coarse levels fused into one launch give the same bits as one launch per
-  layer (outputs and gradients), and match the oracle;
+  layer (outputs and gradients, and the inference path), and match the oracle;
 - a network trained on the GPU on pairs from the plain solver needs fewer
   Krylov iterations than Jacobi-PCG and reaches the SuperLU solution to 1e-8;
 - a broken network triggers the Jacobi-PCG fallback and Newton still converges;
@@ -146,6 +146,33 @@ def test_fused_coarse_tail_is_bit_identical_to_per_layer_launches(torch_gpu, H, 
             gw, gw_ref = g1[w_off:b_off + co], g_ref[w_off:b_off + co]
             err = np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30)
             assert err < 5e-2, (name, err)
+
+
+@pytest.mark.parametrize("n,relu", [(128, 0), (100, 1)])
+def test_fused_inference_apply_is_bit_identical(torch_gpu, n, relu):
+    """apx_unet_apply (the inference path) with the coarse levels in one launch gives the same
+    bits as one launch per layer, on a square grid (129 nodes) and one whose coarse grids have
+    even sides."""
+    torch = torch_gpu
+    from autopdex_amd.learned import UNetPreconditioner, default_levels
+    print(f"[case] fused apply n={n} relu={relu}", flush=True)
+    _, _, _, _, prob = _poisson_problem(n)
+    L = default_levels(n + 1, n + 1)
+    ch = ((16, 32) + (64,) * 16)[:L]
+    net = UNetPreconditioner(n + 1, n + 1, L, ch, 1, 1, relu=bool(relu), batch=1, seed=9)
+    rng = np.random.default_rng(5)
+    theta = net.get_params()
+    for ly in net.layers:
+        theta[ly["b_off"]:ly["b_off"] + ly["co"]] = 0.05 * rng.standard_normal(ly["co"])
+    net.set_params(theta)
+    net.attach(prob)
+    r = torch.from_numpy(rng.standard_normal(prob.n_free)).cuda()
+    zs = {}
+    for fused in (True, False):
+        net.set_fused_tail(fused)
+        zs[fused] = net.apply(prob, r).cpu().numpy()
+    assert np.isfinite(zs[True]).all() and np.abs(zs[True]).max() > 0
+    assert np.array_equal(zs[True].view(np.uint64), zs[False].view(np.uint64))
 
 
 def _poisson_problem(n):
