@@ -95,9 +95,15 @@ int wgrad_grid(const WgradArgs& a) {
 
 void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
   if (f32) {
-    if (ci == 16 && co == 1) return launch_conv_t<16, 1, true>(a, st);
-    if (ci == 16 && co == 2) return launch_conv_t<16, 2, true>(a, st);
-    throw Error(APX_ERR_UNSUPPORTED, "FP32-output convolution only for the 16 -> 1|2 output layer");
+    const int64_t tiles = (int64_t)a.B * ((a.W + 63) / 64) * ((a.H + 3) / 4);
+    APX_REQUIRE(tiles < INT32_MAX, APX_ERR_UNSUPPORTED, "output layer: grid too large");
+    if (ci == 16 && co == 1)
+      hipLaunchKernelGGL(k_conv_out<1>, dim3((int)tiles), dim3(256), 0, st, a);
+    else if (ci == 16 && co == 2)
+      hipLaunchKernelGGL(k_conv_out<2>, dim3((int)tiles), dim3(256), 0, st, a);
+    else
+      throw Error(APX_ERR_UNSUPPORTED, "FP32-output convolution only for the 16 -> 1|2 output layer");
+    return;
   }
   if (ci == 16 && co == 16) return launch_conv_ws<16, 16>(a, st);
   if (ci == 16 && co == 32) return launch_conv_ws<16, 32>(a, st);

@@ -433,6 +433,86 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
   }
 }
 
+// ---- output layer ------------------------------------------------------------------
+// 16 -> CO channels (CO = 1 or 2), FP32 output scaled per image:
+//   y[p, co] = s_img act(b[co] + sum_t Z[p + o_t, 9 co + t]),  Z[q, 9 co + t] = sum_c X[q, c] W[co][16 t + c].
+// The per-tap partials Z of every node of a TR x 64 tile plus its one-node halo come from one
+// MFMA per 16 nodes and 16 partial columns (A = 16 nodes x 16 channels, loaded straight from
+// HBM: 8 B per lane, 512 B contiguous per 16 nodes; B = the 16 x 9 CO weights); each output
+// then sums its nine neighbours' partials from LDS.  The implicit GEMM of k_conv3x3 would use
+// CO of its 16 output columns: this form does 9x fewer MFMAs and stages no input in LDS.
+template <int CO>
+__global__ __launch_bounds__(256) void k_conv_out(ConvArgs a) {
+  constexpr int NN = 9 * CO;                // partial columns
+  constexpr int NT = (NN + 15) / 16;        // MFMA N-tiles
+  constexpr int ZS = NN + 1;                // LDS floats per staged node (odd: no bank conflicts)
+  constexpr int TR = 4, SR = TR + 2;        // output rows, staged rows
+  constexpr int SG = 5, SC = 16 * SG;       // 16-node groups per staged row, staged columns (>= 66)
+  constexpr int NG = SR * SG, GPW = (NG + 3) / 4;  // groups, groups per wavefront
+  __shared__ float sz[SR * SC * ZS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+  const int tiles_x = (a.W + 63) / 64, tiles_img = tiles_x * ((a.H + TR - 1) / TR);
+  const int img = blockIdx.x / tiles_img, trem = blockIdx.x - img * tiles_img;
+  const int ty0 = (trem / tiles_x) * TR, tx0 = (trem % tiles_x) * 64;
+  // B fragments: lane (lr, lq) holds W[co][16 t + 4 lq + j] of column n = 16 nt + lr = 9 co + t
+  bf16x4 wb[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int n = min(16 * nt + lr, NN - 1);
+    wb[nt] = *(const bf16x4*)(a.w + (n / 9) * 144 + (n % 9) * 16 + 4 * lq);
+    if (16 * nt + lr >= NN) wb[nt] = bf16x4{0, 0, 0, 0};
+  }
+  // A fragments of this wavefront's groups g = wave + 4 k: node lr of the group, channels 4 lq..
+  // (unconditional loads, clamped into the image; nodes outside contribute zero partials)
+  bf16x4 xa[GPW];
+  uint32_t in = 0;
+#pragma unroll
+  for (int k = 0; k < GPW; ++k) {
+    const int g = min(wave + 4 * k, NG - 1);
+    const int yy = ty0 - 1 + g / SG, xx = tx0 - 1 + 16 * (g % SG) + lr;
+    in |= (uint32_t)(wave + 4 * k < NG && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) << k;
+    const int yc = min(max(yy, 0), a.H - 1), xc = min(max(xx, 0), a.W - 1);
+    xa[k] = *(const bf16x4*)(a.x + (((int64_t)img * a.H + yc) * a.W + xc) * 16 + 4 * lq);
+  }
+  float bias[CO];
+#pragma unroll
+  for (int co = 0; co < CO; ++co) bias[co] = a.bias ? a.bias[co] : 0.f;
+  const float scale = a.yscale ? a.yscale[img] : 1.f;
+  // stop flag: uniform over the grid (set before the launch); read after the loads are issued
+  if (a.stop && *a.stop) return;
+#pragma unroll
+  for (int k = 0; k < GPW; ++k) {
+    const int g = wave + 4 * k;
+    if (g < NG) {
+      const bf16x4 af = ((in >> k) & 1u) ? xa[k] : bf16x4{0, 0, 0, 0};
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, wb[nt], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        const int n = 16 * nt + lr;  // D: node 4 lq + j of the group, column n
+        if (n < NN) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sz[((g / SG) * SC + 16 * (g % SG) + 4 * lq + j) * ZS + n] = d[j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 6, x = threadIdx.x & 63;  // output node (ty0 + r, tx0 + x)
+  const int yy = ty0 + r, xx = tx0 + x;
+  if (yy < a.H && xx < a.W) {
+    float* dst = (float*)a.y + (((int64_t)img * a.H + yy) * a.W + xx) * CO;
+#pragma unroll
+    for (int co = 0; co < CO; ++co) {
+      float v = 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) v += sz[((r + t / 3) * SC + x + t % 3) * ZS + 9 * co + t];
+      v += bias[co];
+      if (a.relu) v = fmaxf(v, 0.f);
+      dst[co] = v * scale;
+    }
+  }
+}
+
 // ---- weight gradient ---------------------------------------------------------
 // dW[t, co, ci] = sum_p dY[p, co] X[p + o_t, ci]   (dY masked by ymask > 0 when given)
 // db[co] = sum_p dY[p, co].  Each block accumulates over its tiles and writes
