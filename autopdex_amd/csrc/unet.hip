@@ -308,9 +308,9 @@ __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ f
 // One workgroup per image runs the steps in order, with a block barrier after each (the
 // workgroup's global stores are visible to all its threads after the barrier).
 // Convolution steps form the same products in the same accumulation order as
-// k_conv3x3<64, 64, false, NB> (A = 16 pixels x 16 K, B = 16 K x 16 output channels, K
-// blocks tap-major, 16 channels each), with the same epilogue, so the results are
-// bit-identical to the per-layer launches.  Pixels are grouped 16 at a time in row-major
+// k_conv3x3<64, 64, false, NB> (A = 16 pixels x 32 K, B = 32 K x 16 output channels, K
+// blocks tap-major, 32 channels each, on the 16x16x32 MFMA), with the same epilogue, so the
+// results are bit-identical to the per-layer launches.  Pixels are grouped 16 at a time in row-major
 // order over the whole grid; the input is staged in LDS with a zero halo ((H + 2) x (W + 2)
 // nodes, pixel stride kTailS), the weights as [co][k] rows of stride kTailSW.  Wavefront w
 // computes output channels 32 (w & 1) .. 32 (w & 1) + 31 of the pixel groups w / 2,
@@ -344,23 +344,23 @@ __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint1
 #pragma unroll
   for (int mm = 0; mm < NM; ++mm) {
     const int p = min((m0 + 4 * mm) * 16 + lr, HW - 1);  // lanes past the grid repeat its last node
-    lp[mm] = ((p / W) * WP + p % W) * kTailS + 4 * lg;   // top-left node of the 3 x 3 window
+    lp[mm] = ((p / W) * WP + p % W) * kTailS + 8 * lg;   // top-left node of the 3 x 3 window
   }
-  const uint16_t* wb0 = sw + (32 * cp + lr) * kTailSW + 4 * lg;
+  const uint16_t* wb0 = sw + (32 * cp + lr) * kTailSW + 8 * lg;
   const uint16_t* wb1 = wb0 + 16 * kTailSW;
   f32x4 acc[NM][2];
 #pragma unroll
   for (int mm = 0; mm < NM; ++mm) acc[mm][0] = acc[mm][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kb = 0; kb < 36; ++kb) {
-    const int t = kb >> 2;  // tap 3 dy + dx, then 16-channel block kb & 3
-    const int off = ((t / 3) * WP + t % 3) * kTailS + (kb & 3) * 16;
-    const bf16x4 b0 = *(const bf16x4*)(wb0 + kb * 16), b1 = *(const bf16x4*)(wb1 + kb * 16);
+  for (int kb = 0; kb < 18; ++kb) {
+    const int t = kb >> 1;  // tap 3 dy + dx, then 32-channel block kb & 1
+    const int off = ((t / 3) * WP + t % 3) * kTailS + (kb & 1) * 32;
+    const bf16x8 b0 = *(const bf16x8*)(wb0 + kb * 32), b1 = *(const bf16x8*)(wb1 + kb * 32);
 #pragma unroll
     for (int mm = 0; mm < NM; ++mm) {
-      const bf16x4 af = *(const bf16x4*)(sx + lp[mm] + off);
-      acc[mm][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, b0, acc[mm][0], 0, 0, 0);
-      acc[mm][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, b1, acc[mm][1], 0, 0, 0);
+      const bf16x8 af = *(const bf16x8*)(sx + lp[mm] + off);
+      acc[mm][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b0, acc[mm][0], 0, 0, 0);
+      acc[mm][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b1, acc[mm][1], 0, 0, 0);
     }
   }
 #pragma unroll

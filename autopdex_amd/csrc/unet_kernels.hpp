@@ -4,7 +4,8 @@
 // Tensors are NHWC (channel fastest), batch of B images of H x W nodes,
 // activations and activation gradients stored as bf16, accumulation in FP32
 // on the matrix cores (v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j],
-// B[4(l/16)+j][l%16] and D[4(l/16)+j][l%16]).
+// B[4(l/16)+j][l%16] and D[4(l/16)+j][l%16]; layers with 32 or 64 input channels use
+// v_mfma_f32_16x16x32_bf16, whose lanes hold 8 K elements of A and B and D as above).
 //
 // 3x3 convolution (stride 1, zero padding) as implicit GEMM:
 //   Y[p, co] = act( sum_{t<9} sum_ci X[p + o_t, ci] W[t, co, ci] + b[co] )
@@ -25,6 +26,7 @@ namespace apx {
 namespace nn {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));  // operands of the 16x16x32 MFMA
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
@@ -198,6 +200,25 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int n = 0; n < NB; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (CI % 32 == 0) {
+      // 32-wide K blocks (tap-major, 32 channels each) on the double-K MFMA: lane (lr, lg) holds
+      // K elements 8 lg .. 8 lg + 7 of both operands, 16-byte LDS reads
+#pragma unroll
+      for (int kb = 0; kb < 9 * CI / 32; ++kb) {
+        bf16x8 bfrag[NB];
+#pragma unroll
+        for (int n = 0; n < NB; ++n) bfrag[n] = *(const bf16x8*)(sw + (n * 16 + lr) * SWK + kb * 32 + 8 * lg);
+        const int t = kb / (CI / 32), cb = (kb % (CI / 32)) * 32;
+        const int dy = t / 3, dx = t % 3;  // halo offsets 0..2
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const bf16x8 af =
+              *(const bf16x8*)(sx + ((wave + dy) * (kTileW + 2) + m * 16 + lr + dx) * S + cb + 8 * lg);
+#pragma unroll
+          for (int n = 0; n < NB; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfrag[n], acc[m][n], 0, 0, 0);
+        }
+      }
+    } else
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
       bf16x4 bfrag[NB];
