@@ -27,6 +27,11 @@ namespace nn {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));  // operands of the 16x16x32 MFMA
+typedef short short8 __attribute__((ext_vector_type(8)));
+// two 4-element fragments as one 8-element operand (lo: elements 0..3)
+__device__ __forceinline__ bf16x8 pair8(bf16x4 lo, bf16x4 hi) {
+  return __builtin_bit_cast(bf16x8, (short8)__builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
@@ -416,20 +421,38 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
 #pragma unroll
     for (int yi = 0; yi < TR + 2; ++yi)
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const bf16x4 f = *(const bf16x4*)(sx + ((4 * c + lq) * PP + yi * TWP + 16 * wave + lr + dx) * 4);
+      for (int dx = 0; dx < 3; ++dx) {
+        if constexpr (CI == 32) {
+          // one 16x16x32 MFMA per tap: lane group lq holds channels 4 lq .. 4 lq + 3 (plane lq)
+          // and 16 + 4 lq .. 16 + 4 lq + 3 (plane lq + 4) of both operands
+          const int o = (yi * TWP + 16 * wave + lr + dx) * 4;
+          const bf16x8 f = pair8(*(const bf16x4*)(sx + lq * PP * 4 + o), *(const bf16x4*)(sx + (lq + 4) * PP * 4 + o));
 #pragma unroll
           for (int dy = 0; dy < 3; ++dy) {
             const int r = yi - dy;  // output row fed by staged row yi through tap row dy
             if (r >= 0 && r < TR) {
 #pragma unroll
               for (int n = 0; n < NO; ++n)
-                acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[dy * 3 + dx][c][n], f, acc[r][n], 0, 0, 0);
+                acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pair8(wa[dy * 3 + dx][0][n], wa[dy * 3 + dx][1][n]),
+                                                                    f, acc[r][n], 0, 0, 0);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const bf16x4 f = *(const bf16x4*)(sx + ((4 * c + lq) * PP + yi * TWP + 16 * wave + lr + dx) * 4);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+              const int r = yi - dy;  // output row fed by staged row yi through tap row dy
+              if (r >= 0 && r < TR) {
+#pragma unroll
+                for (int n = 0; n < NO; ++n)
+                  acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[dy * 3 + dx][c][n], f, acc[r][n], 0, 0, 0);
+              }
             }
           }
         }
+      }
     const int xx = tx0 + 16 * wave + lr;
     if (xx < a.W) {
       uint16_t* dst0 = (uint16_t*)a.y + (((int64_t)img * a.H + ty0) * a.W + xx) * CO + 4 * lq;
