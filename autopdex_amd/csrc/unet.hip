@@ -199,7 +199,7 @@ template <int C8>
 __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict__ coarse, const uint16_t* skip,
                                                      uint16_t* out, int H, int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= Wc * C8) return;
+  if (idx >= Wc * C8 || (stop && *stop)) return;
   const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
   const uint16_t* cb = coarse + (size_t)Hc * Wc * C * b + c;
@@ -220,7 +220,6 @@ __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict_
     s10 = *(const u32x4*)(skip + o10);
     s11 = *(const u32x4*)(skip + o11);
   }
-  if (stop && *stop) return;  // tested after the loads are issued: the round trips overlap
   float t00[8], t01[8], t10[8], t11[8], sk[8], v[8];
   bf8_unpack(q00, t00);
   bf8_unpack(q01, t01);
@@ -296,11 +295,10 @@ template <int C8>
 __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ fine, uint16_t* __restrict__ out, int H,
                                                   int W, int Hc, int Wc, const int* stop) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= Wc * C8) return;
+  if (idx >= Wc * C8 || (stop && *stop)) return;
   const int J = idx / C8, c = (idx % C8) * 8, I = blockIdx.y, b = blockIdx.z;
   constexpr int C = 8 * C8;
   const u32x4 v = restrict_chunk<C>(fine + (size_t)H * W * C * b, I, J, c, H, W);
-  if (stop && *stop) return;  // tested after the loads are issued: the round trips overlap
   *(u32x4*)(out + (((size_t)b * Hc + I) * Wc + J) * C + c) = v;
 }
 
@@ -380,6 +378,7 @@ __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint1
 }
 
 __global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const int* stop) {
+  if (stop && *stop) return;  // uniform over the grid: set before the launch
   __shared__ __attribute__((aligned(16))) uint16_t sx[kTailPix * kTailS];
   __shared__ __attribute__((aligned(16))) uint16_t sw[64 * kTailSW];
   const int img = blockIdx.x;
@@ -394,9 +393,6 @@ __global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const
     }
   };
   fetch_weights(args.w_first);
-  // stop flag: uniform over the grid (set before the launch); read after the first loads are
-  // issued, so that its round trip overlaps theirs
-  if (stop && *stop) return;
   for (int si = 0; si < args.n; ++si) {
     const TailStep& st = args.step[si];
     if (st.op == kTailConv) {

@@ -103,6 +103,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   static_assert(OUT_F32 || CB % 8 == 0, "bf16 outputs are stored in 8-channel chunks");
   __shared__ __attribute__((aligned(16))) uint16_t sbuf[SBUF];
   __shared__ __attribute__((aligned(16))) uint16_t sw[16 * NB * SWK];
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   uint16_t* sx = sbuf;
   const int co0 = blockIdx.y * 16 * NB;  // first output channel of this block
   // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; the
@@ -179,9 +180,6 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   };
   const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
   if (tile0 < ntiles) load_tile_any(tile0);
-  // stop flag: uniform over the grid (set before the launch); read after the first loads are
-  // issued, so that its round trip overlaps theirs
-  if (a.stop && *a.stop) return;
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int i = threadIdx.x + r * kConvThreads;
@@ -332,6 +330,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
   constexpr int RC = (NL + kConvThreads - 1) / kConvThreads;
   constexpr int WK = 9 * CI;                 // weight row length
   __shared__ __attribute__((aligned(16))) uint16_t sx[(CI / 4) * PP * 4];
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lq = lane >> 4;
   bf16x4 wa[9][NC][NO];
@@ -396,9 +395,6 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
   };
   const int tile0 = (int)xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
   if (tile0 < ntiles) load_tile(tile0);
-  // stop flag: uniform over the grid (set before the launch); read after the first loads are
-  // issued, so that its round trip overlaps theirs
-  if (a.stop && *a.stop) return;
   for (int tile = tile0; tile < ntiles; tile += gridDim.x) {
     const int img = tile / tiles_img, trem = tile - img * tiles_img;
     const int tyi = trem / tiles_x;
@@ -494,6 +490,7 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a) {
   constexpr int SG = 5, SC = 16 * SG;       // 16-node groups per staged row, staged columns (>= 66)
   constexpr int NG = SR * SG, GPW = (NG + 3) / 4;  // groups, groups per wavefront
   __shared__ float sz[SR * SC * ZS];
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
   const int tiles_x = (a.W + 63) / 64, tiles_img = tiles_x * ((a.H + TR - 1) / TR);
   const int img = blockIdx.x / tiles_img, trem = blockIdx.x - img * tiles_img;
@@ -522,8 +519,6 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a) {
 #pragma unroll
   for (int co = 0; co < CO; ++co) bias[co] = a.bias ? a.bias[co] : 0.f;
   const float scale = a.yscale ? a.yscale[img] : 1.f;
-  // stop flag: uniform over the grid (set before the launch); read after the loads are issued
-  if (a.stop && *a.stop) return;
 #pragma unroll
   for (int k = 0; k < GPW; ++k) {
     const int g = wave + 4 * k;
