@@ -989,13 +989,14 @@ __global__ void k_to_grid(int64_t n_nodes, int d, int cin, const uint8_t* dmask,
        node += (int64_t)gridDim.x * blockDim.x) {
     uint16_t* xo = x + ((int64_t)img * n_nodes + node) * cin;
     for (int c = 0; c < d; ++c) {
+      // loads unconditional (a Dirichlet DOF reads reduced entry 0 and stores zero), so that
+      // they are not serialised behind a per-lane branch
       const int64_t gd = node * d + c;
-      if (dmask[gd]) {
-        xo[c] = 0;
-      } else {
-        const int32_t i = full2red[gd];
-        xo[c] = f2bf((float)((double)r[i] * (dsq ? dsq[i] : 1.0)) * inv);
-      }
+      const bool fixed = dmask[gd] != 0;
+      const int32_t i0 = full2red[gd];
+      const int32_t i = fixed ? 0 : i0;
+      const float v = (float)((double)r[i] * (dsq ? dsq[i] : 1.0)) * inv;
+      xo[c] = fixed ? (uint16_t)0 : f2bf(v);
     }
     if (cin > d) xo[d] = f2bf(g ? (float)g[node] : 0.f);
   }
@@ -1042,6 +1043,7 @@ void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, cons
   APX_REQUIRE((int64_t)N.H[0] * N.W[0] == P.n_nodes, APX_ERR_ARG, "network grid does not match the plan's nodes");
   APX_REQUIRE(N.c_in == P.d || N.c_in == P.d + 1, APX_ERR_ARG, "network input channels do not match the DOFs");
   APX_REQUIRE(img >= 0 && img < N.batch, APX_ERR_ARG, "image index out of range");
+  APX_REQUIRE(P.n_free > 0, APX_ERR_ARG, "network input: the plan has no free DOFs");
   ensure_krylov_scratch(P, st);
   const double* dsq = P.net_dsq_set ? P.net_dsq.p : nullptr;
   if (dsq) rr_src = nullptr;  // the scale is the RMS of the scaled input
