@@ -109,12 +109,19 @@ void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
   if (ci == 16 && co == 32) return launch_conv_ws<16, 32>(a, st);
   if (ci == 32 && co == 16) return launch_conv_ws<32, 16>(a, st);
   if (ci == 32 && co == 32) return launch_conv_ws<32, 32>(a, st);
+  if ((ci == 1 || ci == 2) && co == 16 && !a.xmask) {  // the input layer
+    const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
+    APX_REQUIRE(tiles < INT32_MAX, APX_ERR_UNSUPPORTED, "input layer: grid too large");
+    if (ci == 1)
+      hipLaunchKernelGGL(k_conv_in<1>, dim3((int)tiles), dim3(kConvThreads), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_conv_in<2>, dim3((int)tiles), dim3(kConvThreads), 0, st, a);
+    return;
+  }
 #define APX_X(A, B) \
   if (ci == A && co == B) return launch_conv_t<A, B, false>(a, st);
   APX_CONV_CASES(APX_X)
 #undef APX_X
-  if (ci == 1 && co == 16) return launch_conv_t<1, 16, false>(a, st);
-  if (ci == 2 && co == 16) return launch_conv_t<2, 16, false>(a, st);
   throw Error(APX_ERR_UNSUPPORTED, "no convolution kernel for " + std::to_string(ci) + " -> " + std::to_string(co) +
                                        " channels");
 }

@@ -552,6 +552,77 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a) {
   }
 }
 
+// ---- input layer -------------------------------------------------------------------
+// CI = 1 or 2 -> 16 channels (bf16 output), weight-stationary like k_conv3x3_ws: A = W (16
+// output channels x 16 K, packed k = t CI + c, K blocks of 16 as in k_conv3x3), B = 16 pixels'
+// 3 x 3 patches read from the staged input tile in LDS; lane (lr, lq) ends with channels
+// 4 lq .. 4 lq + 3 of pixel lr, stored straight to HBM (8 B per lane, 512 B contiguous per 16
+// pixels; the layer writes 16x what it reads).  One 4 x 64 tile per block, wavefront w owns
+// pixel columns 16 w .. 16 w + 15.
+template <int CI>
+__global__ __launch_bounds__(kConvThreads) void k_conv_in(ConvArgs a) {
+  constexpr int KB = (9 * CI + 15) / 16;
+  constexpr int TR = kTileH, SR = TR + 2, SW = kTileW + 2;  // output rows, staged rows and columns
+  constexpr int NS = SR * SW * CI, RS = (NS + kConvThreads - 1) / kConvThreads;
+  __shared__ uint16_t sx[RS * kConvThreads];
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+  const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_img = tiles_x * ((a.H + TR - 1) / TR);
+  const int tile = (int)xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
+  const int img = tile / tiles_img, trem = tile - img * tiles_img;
+  const int ty0 = (trem / tiles_x) * TR, tx0 = (trem % tiles_x) * kTileW;
+  const uint16_t* xb = a.x + (int64_t)img * a.H * a.W * CI;
+  // staged element i = (row, column, channel), row-major; unconditional clamped loads, zero outside
+  uint16_t q[RS];
+  uint32_t qin = 0;
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int i = threadIdx.x + r * kConvThreads;
+    const int c = i % CI, px = (i / CI) % SW, row = i / (CI * SW);
+    const int yy = ty0 - 1 + row, xx = tx0 - 1 + px;
+    qin |= (uint32_t)(i < NS && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) << r;
+    const int yc = min(max(yy, 0), a.H - 1), xc = min(max(xx, 0), a.W - 1);
+    q[r] = xb[((int64_t)yc * a.W + xc) * CI + c];
+  }
+  bf16x4 wa[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) wa[kb] = *(const bf16x4*)(a.w + lr * (KB * 16) + kb * 16 + 4 * lq);
+  float bias[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bias[j] = a.bias ? a.bias[4 * lq + j] : 0.f;
+#pragma unroll
+  for (int r = 0; r < RS; ++r) sx[threadIdx.x + r * kConvThreads] = ((qin >> r) & 1u) ? q[r] : (uint16_t)0;
+  __syncthreads();
+  const int xo = 16 * wave + lr;  // this lane's pixel column in the tile
+#pragma unroll
+  for (int r = 0; r < TR; ++r) {
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      bf16x4 b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 16 * kb + 4 * lq + j, kc = min(k, 9 * CI - 1);
+        const int t = kc / CI, c = kc % CI;
+        const uint16_t v = sx[((r + t / 3) * SW + xo + t % 3) * CI + c];
+        b[j] = k < 9 * CI ? (short)v : (short)0;
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[kb], b, acc, 0, 0, 0);
+    }
+    const int yy = ty0 + r, xx = tx0 + xo;
+    if (yy < a.H && xx < a.W) {
+      float v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = acc[j] + bias[j];
+        if (a.relu) v[j] = fmaxf(v[j], 0.f);
+      }
+      *(uint2*)((uint16_t*)a.y + (((int64_t)img * a.H + yy) * a.W + xx) * 16 + 4 * lq) =
+          uint2{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])};
+    }
+  }
+}
+
 // ---- weight gradient ---------------------------------------------------------
 // dW[t, co, ci] = sum_p dY[p, co] X[p + o_t, ci]   (dY masked by ymask > 0 when given)
 // db[co] = sum_p dY[p, co].  Each block accumulates over its tiles and writes
