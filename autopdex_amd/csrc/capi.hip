@@ -417,7 +417,7 @@ int apx_unet_apply(apx_unet* net, apx_plan* plan, const double* r_dev, const dou
     APX_REQUIRE(r_dev && z_dev, APX_ERR_ARG, "null device pointer");
     APX_REQUIRE(N.batch == 1, APX_ERR_ARG, "apply needs a batch-1 network");
     net_input_from_reduced_img(P, N, r_dev, 1, g_nodal_dev, nullptr, 0, S(stream));
-    N.forward(S(stream));
+    N.forward(S(stream), false);
     net_output_to_reduced_img(P, N, r_dev, z_dev, 1, 0, S(stream));
   });
 }

@@ -217,7 +217,7 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   auto iteration = [&] {
     EventPair e0 = timing_begin(P, st);
     net_input_from_reduced_img(P, N, r, 1, g_nodal, &S->rr, 0, st);
-    N.forward(st);
+    N.forward(st, false);
     hipLaunchKernelGGL(k_fcg_z, dim3(gv), dim3(kBS), 0, st, n, P.red2full.p, N.y.p, P.net_dsq.p,
                        N.jacobi_skip ? r : nullptr, N.scale.p, q, z, S, P.partials.p);
     timing_end(P, e0, st);

@@ -61,14 +61,16 @@ void launch_conv_t(const ConvArgs& a, hipStream_t st) {
   }
 }
 
-template <int CI, int CO, int TR>
+template <int CI, int CO, int TR, int IN = 0>
 void launch_conv_ws_t(const ConvArgs& a, hipStream_t st) {
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + TR - 1) / TR);
   APX_REQUIRE(tiles < (int64_t(1) << 30) && (int64_t)(TR + 2) * a.W * CI < (int64_t(1) << 31), APX_ERR_UNSUPPORTED,
               "convolution grid too large for 32-bit tile indices");
-  static const int64_t cap = resident_blocks(k_conv3x3_ws<CI, CO, TR>);
+  APX_REQUIRE(IN == 0 || (a.xin && a.win && !a.xmask && (int64_t)a.H * a.W * IN < INT32_MAX), APX_ERR_UNSUPPORTED,
+              "convolution of the input layer's output: no image, or the image is too large");
+  static const int64_t cap = resident_blocks(k_conv3x3_ws<CI, CO, TR, IN>);
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
-  hipLaunchKernelGGL((k_conv3x3_ws<CI, CO, TR>), dim3(g), dim3(kConvThreads), 0, st, a);
+  hipLaunchKernelGGL((k_conv3x3_ws<CI, CO, TR, IN>), dim3(g), dim3(kConvThreads), 0, st, a);
 }
 
 // Four-row tiles: measured faster than eight-row tiles (which hold twice the accumulators and
@@ -803,6 +805,36 @@ void conv_wgrad(UNet& N, int li, int level, const uint16_t* x, const uint16_t* d
                      ly.cop, ly.wk, ly.co, N.grad.p, ly.w_off, ly.b_off);
 }
 
+// d_0 = conv_e[0](conv_in(x0)) without storing conv_in's output: the input layer is formed while
+// staging the level-0 tiles (bit-identical to the two launches)
+void conv_fwd_in_e0(UNet& N, hipStream_t st) {
+  const NetLayer& li = N.layers[N.L_in];
+  const NetLayer& le = N.layers[N.L_e[0]];
+  ConvArgs a{};
+  a.x = nullptr;
+  a.xmask = nullptr;
+  a.xin = N.x0.p;
+  a.win = N.wf.p + li.wf_off;
+  a.bin = N.theta.p + li.b_off;
+  a.relu_in = li.relu;
+  a.w = N.wf.p + le.wf_off;
+  a.bias = N.theta.p + le.b_off;
+  a.y = N.d[0].p;
+  a.yscale = nullptr;
+  a.B = N.batch;
+  a.H = N.H[0];
+  a.W = N.W[0];
+  a.relu = le.relu;
+  a.stop = N.stop;
+  APX_REQUIRE(li.co == 16 && le.ci == 16 && le.co == 16, APX_ERR_UNSUPPORTED, "fused input layer: 16 channels");
+  if (N.c_in == 1)
+    launch_conv_ws_t<16, 16, 4, 1>(a, st);
+  else if (N.c_in == 2)
+    launch_conv_ws_t<16, 16, 4, 2>(a, st);
+  else
+    throw Error(APX_ERR_UNSUPPORTED, "fused input layer: 1 or 2 input channels");
+}
+
 void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const uint16_t* skip, uint16_t* out,
                  hipStream_t st) {
   const int H = N.H[level_fine], W = N.W[level_fine];
@@ -855,11 +887,11 @@ int64_t UNet::forward_bytes() const {
 
 void UNet::collect_timing(size_t count) {
   if (!timer_plan) return;
-  timing_collect(*timer_plan, "unet_conv_e0", ev_conv0, levels > 1 ? conv_bytes(L_e[0], 0) : 0, count);
+  timing_collect(*timer_plan, "unet_conv_d0", ev_conv0, levels > 1 ? conv_bytes(L_d[0], 0) : 0, count);
   timing_collect(*timer_plan, "unet_forward", ev_fwd, forward_bytes(), count);
 }
 
-void UNet::forward(hipStream_t st) {
+void UNet::forward(hipStream_t st, bool keep_activations) {
   APX_REQUIRE(weights_ready, APX_ERR_STATE, "network weights not prepared");
   const int L = levels;
   const bool timed = timer_plan && timer_plan->timing;
@@ -867,15 +899,15 @@ void UNet::forward(hipStream_t st) {
   // levels >= c from the restriction of level c to its prolongation: one launch (the tail)
   const bool fused = fuse_tail && tail_level >= 0;
   const int c = fused ? tail_level : L;
-  conv_fwd(*this, L_in, 0, x0.p, a[0].p, st);
+  // inference: the input layer is formed inside the level-0 conv_e launch (its output a_0 is
+  // read by nothing else; training keeps it for the backward pass)
+  const bool fused_in = fuse_tail && !keep_activations && L > 1 && ch[0] == 16;
+  if (fused_in)
+    conv_fwd_in_e0(*this, st);
+  else
+    conv_fwd(*this, L_in, 0, x0.p, a[0].p, st);
   for (int l = 0; l + 1 < L && l <= c; ++l) {
-    EventPair ec{};
-    if (timed && l == 0) ec = timing_begin(*timer_plan, st);
-    conv_fwd(*this, L_e[l], l, a[l].p, d[l].p, st);
-    if (timed && l == 0) {
-      timing_end(*timer_plan, ec, st);
-      ev_conv0.push_back(ec);
-    }
+    if (l > 0 || !fused_in) conv_fwd(*this, L_e[l], l, a[l].p, d[l].p, st);
     if (l == c) break;
     restrict_(*this, l, ch[l], d[l].p, rd[l].p, st);
     conv_fwd(*this, L_down[l], l + 1, rd[l].p, a[l + 1].p, st);
@@ -889,7 +921,13 @@ void UNet::forward(hipStream_t st) {
       conv_fwd(*this, L_up[l], l + 1, u[l + 1].p, t[l].p, st);
       prolong_add(*this, l, ch[l], t[l].p, d[l].p, s[l].p, st);
     }
+    EventPair ec{};
+    if (timed && l == 0) ec = timing_begin(*timer_plan, st);
     conv_fwd(*this, L_d[l], l, s[l].p, u[l].p, st);
+    if (timed && l == 0) {
+      timing_end(*timer_plan, ec, st);
+      ev_conv0.push_back(ec);
+    }
   }
   conv_fwd(*this, L_out, 0, u[0].p, y.p, st);
   APX_LAUNCH_CHECK();

@@ -66,7 +66,7 @@ struct UNet {
   bool weights_ready = false;
   // launch timing (HIP events on the launch stream) when the attached plan times its kernels
   struct Plan* timer_plan = nullptr;
-  std::vector<EventPair> ev_conv0, ev_fwd;
+  std::vector<EventPair> ev_conv0, ev_fwd;  // level-0 conv_d launches, forward passes
   // optional device stop flag (a Krylov solver's done flag): forward-pass kernels launched
   // after it is set return at once
   const int* stop = nullptr;
@@ -81,7 +81,9 @@ struct UNet {
   void build(int levels, const int* channels, int c_in, int c_out, int relu, int batch, int H0, int W0);
   void build_tail();                      // tail_level and the tail's step list (after build's allocations)
   void prepare_weights(hipStream_t st);   // FP32 master -> bf16 forward / backward-data layouts
-  void forward(hipStream_t st);           // x0 -> y
+  // x0 -> y; keep_activations = false (inference): the input layer's output, read only by the
+  // backward pass otherwise, is formed inside the level-0 conv_e launch and not stored
+  void forward(hipStream_t st, bool keep_activations = true);
   // loss = mean_b ||y_b - t_b||^2 / ||t_b||^2 ; fills grad (FP32) via backward
   double loss_backward(const float* target, hipStream_t st);
   // backward pass from the output gradient gy (bf16) into grad

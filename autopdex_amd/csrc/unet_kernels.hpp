@@ -62,6 +62,12 @@ struct ConvArgs {
   int B, H, W;
   int relu;
   const int* stop;        // optional device flag: nonzero -> the launch does nothing
+  // weight-stationary kernel with IN > 0: the input is act(W_in P + b_in) of the IN-channel image
+  // xin (the input layer, formed while staging; x is not read): W_in [16][16 KB_in], b_in [16]
+  const uint16_t* xin;
+  const uint16_t* win;
+  const float* bin;
+  int relu_in;
 };
 
 // ---- forward / backward-data convolution ------------------------------------
@@ -317,9 +323,13 @@ __host__ __device__ constexpr int ws_plane() {
   return (((TR + 2) * (kTileW + 2) + 15) / 32) * 32 + 16;
 }
 
-template <int CI, int CO, int TR>
+// IN > 0 (CI = 16): the staged input tile is the input layer's output, computed from an LDS
+// tile of the IN-channel image with the operands and order of k_conv_in (bit-identical to it),
+// zero outside the image; the input layer's output never goes to HBM.
+template <int CI, int CO, int TR, int IN = 0>
 __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
   static_assert((CI == 16 || CI == 32) && (CO == 16 || CO == 32), "16- or 32-channel layers");
+  static_assert(IN == 0 || CI == 16, "the input layer has 16 output channels");
   constexpr int NC = CI / 16, NO = CO / 16;  // K blocks per tap, output channel blocks
   constexpr int TWP = kTileW + 2;            // staged row length (pixels)
   constexpr int NP = (TR + 2) * TWP;         // staged pixels
@@ -366,6 +376,38 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
     s_lds[r] = used ? ((2 * h) * PP + pix) * 4 : -1;
   }
   u32x4 qs[RC];
+  // IN > 0: the image tile (rows ty0 - 2 .. ty0 + TR + 1, columns tx0 - 2 .. tx0 + 65) in LDS,
+  // element ((row * XW) + column) * IN + channel; bit r of qxin: element r is inside the image
+  constexpr int INC = IN > 0 ? IN : 1, XW = kTileW + 4, XT = (TR + 4) * XW * INC;
+  constexpr int RX = (XT + kConvThreads - 1) / kConvThreads, KBI = (9 * INC + 15) / 16;
+  static_assert(RX <= 32, "one validity bit per staged image element");
+  __shared__ uint16_t xs[IN > 0 ? RX * kConvThreads : 1];
+  uint16_t qx[IN > 0 ? RX : 1];
+  uint32_t qxin = 0;
+  bf16x4 wi[KBI];
+  float bi[4];
+  if constexpr (IN > 0) {
+#pragma unroll
+    for (int kb = 0; kb < KBI; ++kb) wi[kb] = *(const bf16x4*)(a.win + lr * (KBI * 16) + kb * 16 + 4 * lq);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bi[j] = a.bin ? a.bin[4 * lq + j] : 0.f;
+  }
+  auto load_xtile = [&](int tile) {
+    const int img = tile / tiles_img, trem = tile - img * tiles_img;
+    const int tyi = trem / tiles_x;
+    const int ty0 = tyi * TR, tx0 = (trem - tyi * tiles_x) * kTileW;
+    const uint16_t* xb = a.xin + (int64_t)img * a.H * a.W * INC;
+    qxin = 0;
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+      const int i = threadIdx.x + r * kConvThreads;
+      const int c = i % INC, px = (i / INC) % XW, row = i / (INC * XW);
+      const int yy = ty0 - 2 + row, xx = tx0 - 2 + px;
+      qxin |= (uint32_t)(i < XT && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) << r;
+      const int yc = min(max(yy, 0), a.H - 1), xc = min(max(xx, 0), a.W - 1);
+      qx[r] = xb[(yc * a.W + xc) * INC + c];
+    }
+  };
   auto load_tile = [&](int tile) {
     const int img = tile / tiles_img, trem = tile - img * tiles_img;
     const int tyi = trem / tiles_x;
@@ -394,21 +436,69 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_ws(ConvArgs a) {
     }
   };
   const int tile0 = (int)xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
-  if (tile0 < ntiles) load_tile(tile0);
+  if (tile0 < ntiles) {
+    if constexpr (IN > 0)
+      load_xtile(tile0);
+    else
+      load_tile(tile0);
+  }
   for (int tile = tile0; tile < ntiles; tile += gridDim.x) {
     const int img = tile / tiles_img, trem = tile - img * tiles_img;
     const int tyi = trem / tiles_x;
     const int ty0 = tyi * TR, tx0 = (trem - tyi * tiles_x) * kTileW;
     __syncthreads();  // every wavefront is done reading the previous tile
+    if constexpr (IN > 0) {
 #pragma unroll
-    for (int r = 0; r < RC; ++r) {
-      if (s_lds[r] >= 0) {
-        *(uint2*)(sx + s_lds[r]) = uint2{qs[r][0], qs[r][1]};
-        *(uint2*)(sx + s_lds[r] + PP * 4) = uint2{qs[r][2], qs[r][3]};
+      for (int r = 0; r < RX; ++r) xs[threadIdx.x + r * kConvThreads] = ((qxin >> r) & 1u) ? qx[r] : (uint16_t)0;
+      __syncthreads();
+      // the staged tile = the input layer at the staged pixels, 16 pixels per MFMA (D: channels
+      // 4 lq .. 4 lq + 3 of pixel lr = plane lq), zero outside the image
+      constexpr int NG = (NP + 15) / 16;  // 16-pixel groups; wavefront w takes w, w + 4, ...
+#pragma unroll
+      for (int k = 0; k < (NG + 3) / 4; ++k) {
+        const int gi = wave + 4 * k;
+        const int pix = 16 * gi + lr, pc = min(pix, NP - 1);  // past the tile: recomputes its last pixel
+        const int sr = pc / TWP, sc = pc - sr * TWP;
+        f32x4 d = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBI; ++kb) {
+          bf16x4 b;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int k = 16 * kb + 4 * lq + j, kc = min(k, 9 * INC - 1);
+            const int t = kc / INC, c = kc % INC;
+            const uint16_t v = xs[((sr + t / 3) * XW + sc + t % 3) * INC + c];
+            b[j] = k < 9 * INC ? (short)v : (short)0;
+          }
+          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wi[kb], b, d, 0, 0, 0);
+        }
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          v[j] = d[j] + bi[j];
+          if (a.relu_in) v[j] = fmaxf(v[j], 0.f);
+        }
+        const int yy = ty0 - 1 + sr, xx = tx0 - 1 + sc;
+        const bool inside = (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+        if (pix < NP)
+          *(uint2*)(sx + (lq * PP + pix) * 4) = inside ? uint2{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])} : uint2{0u, 0u};
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RC; ++r) {
+        if (s_lds[r] >= 0) {
+          *(uint2*)(sx + s_lds[r]) = uint2{qs[r][0], qs[r][1]};
+          *(uint2*)(sx + s_lds[r] + PP * 4) = uint2{qs[r][2], qs[r][3]};
+        }
       }
     }
     __syncthreads();
-    if (tile + (int)gridDim.x < ntiles) load_tile(tile + (int)gridDim.x);  // in flight during the MFMAs
+    if (tile + (int)gridDim.x < ntiles) {  // in flight during the MFMAs
+      if constexpr (IN > 0)
+        load_xtile(tile + (int)gridDim.x);
+      else
+        load_tile(tile + (int)gridDim.x);
+    }
     f32x4 acc[TR][NO];
 #pragma unroll
     for (int r = 0; r < TR; ++r)

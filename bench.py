@@ -51,7 +51,7 @@ HBM_PEAK = 8.0e12
 # `--pmc WRITE_SIZE` runs of `python bench.py --probe`, scaled by the counters'
 # reading of a 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000).
 PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double>", 2048): 660.2e6, ("k_pcg_spmv<float>", 2048): 408.5e6,
-                     ("k_conv3x3_ws<16, 16, 4>", 2048): 293.4e6}
+                     ("k_conv3x3_ws<16, 16, 4, 0>", 2048): 293.4e6}
 
 
 def build_problem(n):
@@ -156,7 +156,7 @@ def kernel_table(prob):
     out = {}
     for name in ("pcg_spmv_f64", "pcg_update_f64", "pcg_spmv_f32", "pcg_update_f32", "residual", "tangent_f64",
                  "tangent_f64_to_f32", "tangent_f32", "fcg_spmv_f64", "fcg_spmv_f32", "fcg_precond",
-                 "unet_conv_e0", "unet_forward"):
+                 "unet_conv_d0", "unet_forward"):
         ms, cnt, by = timer(prob, name)
         if cnt:
             out[name] = {"avg_ms": ms, "count": cnt, "bytes": by, "GBps": by / (ms * 1e-3) / 1e9 if by else None,
@@ -275,8 +275,8 @@ def main():
     H = args.headline
     if H.startswith("learned"):
         # the heaviest single kernel of flexible CG with the network: the level-0 16 -> 16 convolution
-        dom = kernels["unet_conv_e0"]
-        dom_name = "k_conv3x3_ws<16, 16, 4>"
+        dom = kernels["unet_conv_d0"]
+        dom_name = "k_conv3x3_ws<16, 16, 4, 0>"
     else:
         prec_tag = "f64" if H == "plain" else "f32"
         dom = kernels[f"pcg_spmv_{prec_tag}"]

@@ -251,12 +251,15 @@ int apx_unet_apply(apx_unet* net, apx_plan* plan, const double* r_dev, const dou
  * (enable = 1) instead of z = D^-1/2 s net(D^-1/2 r / s) (0, the default).  Applies to
  * inference, flexible CG and training alike.                                */
 int apx_unet_set_jacobi_skip(apx_unet* net, int enable);
-/* Run the network's coarse levels (from the first restriction into a run of 64-channel
- * levels of at most 9 x 9 nodes down to the matching prolongation) as one launch,
- * one workgroup per image (enable = 1, the default), or one launch per layer (0).  The
- * two give bit-identical outputs.  tail_level (optional) receives the level whose
- * restriction starts the fused part (-1: the network has no such levels).  No
- * counterpart in the reference (a launch-count optimisation of this implementation). */
+/* Fused launches (enable = 1, the default) or one launch per layer (0); the two
+ * give bit-identical outputs.  Fused: the network's coarse levels (from the first
+ * restriction into a run of 64-channel levels of at most 9 x 9 nodes down to the
+ * matching prolongation) run as one launch, one workgroup per image; in inference
+ * (apx_unet_apply, flexible CG) the input layer is formed inside the level-0
+ * conv_e launch and its output is not stored.  tail_level (optional) receives the
+ * level whose restriction starts the fused coarse levels (-1: none).  No
+ * counterpart in the reference (a launch and traffic optimisation of this
+ * implementation).                                                          */
 int apx_unet_set_fused_tail(apx_unet* net, int enable, int* tail_level);
 
 /* Use `net` as preconditioner of the plan (apx_newton with precond

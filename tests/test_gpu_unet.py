@@ -148,29 +148,31 @@ def test_fused_coarse_tail_is_bit_identical_to_per_layer_launches(torch_gpu, H, 
             assert err < 5e-2, (name, err)
 
 
-@pytest.mark.parametrize("n,relu", [(128, 0), (100, 1)])
-def test_fused_inference_apply_is_bit_identical(torch_gpu, n, relu):
-    """apx_unet_apply (the inference path) with the coarse levels in one launch gives the same
-    bits as one launch per layer, on a square grid (129 nodes) and one whose coarse grids have
-    even sides."""
+@pytest.mark.parametrize("n,relu,c_in", [(128, 0, 1), (100, 1, 2)])
+def test_fused_inference_apply_is_bit_identical(torch_gpu, n, relu, c_in):
+    """apx_unet_apply (the inference path) with fused launches (the coarse levels in one launch,
+    the input layer formed inside the level-0 conv_e launch) gives the same bits as one launch
+    per layer, on a square grid (129 nodes) and one whose coarse grids have even sides, with one
+    input channel and with two (residual + nodal field)."""
     torch = torch_gpu
     from autopdex_amd.learned import UNetPreconditioner, default_levels
-    print(f"[case] fused apply n={n} relu={relu}", flush=True)
+    print(f"[case] fused apply n={n} relu={relu} c_in={c_in}", flush=True)
     _, _, _, _, prob = _poisson_problem(n)
     L = default_levels(n + 1, n + 1)
     ch = ((16, 32) + (64,) * 16)[:L]
-    net = UNetPreconditioner(n + 1, n + 1, L, ch, 1, 1, relu=bool(relu), batch=1, seed=9)
+    net = UNetPreconditioner(n + 1, n + 1, L, ch, c_in, 1, relu=bool(relu), batch=1, seed=9)
     rng = np.random.default_rng(5)
     theta = net.get_params()
     for ly in net.layers:
         theta[ly["b_off"]:ly["b_off"] + ly["co"]] = 0.05 * rng.standard_normal(ly["co"])
     net.set_params(theta)
-    net.attach(prob)
+    g = torch.from_numpy(rng.standard_normal((n + 1) ** 2)).cuda() if c_in == 2 else None
+    net.attach(prob, g)
     r = torch.from_numpy(rng.standard_normal(prob.n_free)).cuda()
     zs = {}
     for fused in (True, False):
         net.set_fused_tail(fused)
-        zs[fused] = net.apply(prob, r).cpu().numpy()
+        zs[fused] = net.apply(prob, r, g).cpu().numpy()
     assert np.isfinite(zs[True]).all() and np.abs(zs[True]).max() > 0
     assert np.array_equal(zs[True].view(np.uint64), zs[False].view(np.uint64))
 
