@@ -120,8 +120,21 @@ __global__ __launch_bounds__(kSpmvBS) void k_fcg_spmv(int64_t n, int64_t n_chunk
     const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
     const T* vv = vals + off + lane;
     const int32_t* ci = cols + off + lane;
+    // values and columns are read once: non-temporal; four entries' loads are issued before
+    // their gathers (more loads in flight), the sum keeps the entry order
     double s = 0.0;
-    for (int k = 0; k < w; ++k) {  // values and columns are read once: non-temporal
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+      const int32_t c0 = __builtin_nontemporal_load(ci + (k + 0) * 64), c1 = __builtin_nontemporal_load(ci + (k + 1) * 64),
+                    c2 = __builtin_nontemporal_load(ci + (k + 2) * 64), c3 = __builtin_nontemporal_load(ci + (k + 3) * 64);
+      const T v0 = __builtin_nontemporal_load(vv + (k + 0) * 64), v1 = __builtin_nontemporal_load(vv + (k + 1) * 64),
+              v2 = __builtin_nontemporal_load(vv + (k + 2) * 64), v3 = __builtin_nontemporal_load(vv + (k + 3) * 64);
+      s += (double)v0 * (z[c0] + beta * po[c0]);
+      s += (double)v1 * (z[c1] + beta * po[c1]);
+      s += (double)v2 * (z[c2] + beta * po[c2]);
+      s += (double)v3 * (z[c3] + beta * po[c3]);
+    }
+    for (; k < w; ++k) {
       const int32_t col = __builtin_nontemporal_load(ci + k * 64);
       s += (double)__builtin_nontemporal_load(vv + k * 64) * (z[col] + beta * po[col]);
     }
