@@ -148,12 +148,12 @@ def test_fused_coarse_tail_is_bit_identical_to_per_layer_launches(torch_gpu, H, 
             assert err < 5e-2, (name, err)
 
 
-@pytest.mark.parametrize("n,relu,c_in", [(128, 0, 1), (100, 1, 2)])
+@pytest.mark.parametrize("n,relu,c_in", [(128, 0, 1), (100, 1, 2), (2048, 0, 1)])
 def test_fused_inference_apply_is_bit_identical(torch_gpu, n, relu, c_in):
     """apx_unet_apply (the inference path) with fused launches (the coarse levels in one launch,
     the input layer formed inside the level-0 conv_e launch) gives the same bits as one launch
-    per layer, on a square grid (129 nodes) and one whose coarse grids have even sides, with one
-    input channel and with two (residual + nodal field)."""
+    per layer, on a square grid (129 nodes), one whose coarse grids have even sides, and the
+    2049 x 2049 benchmark grid, with one input channel and with two (residual + nodal field)."""
     torch = torch_gpu
     from autopdex_amd.learned import UNetPreconditioner, default_levels
     print(f"[case] fused apply n={n} relu={relu} c_in={c_in}", flush=True)
