@@ -30,7 +30,7 @@ class NewtonOptions(C.Structure):
                 ("damping", C.c_double), ("precision", C.c_int), ("precond", C.c_int),
                 ("lin_tol", C.c_double), ("lin_atol", C.c_double), ("lin_maxiter", C.c_int),
                 ("reuse_tangent", C.c_int), ("forcing", C.c_int), ("reliable_delta", C.c_double),
-                ("fp32_element_arith", C.c_int)]
+                ("fp32_element_arith", C.c_int), ("krylov_fp64_vectors", C.c_int)]
 
 
 class NewtonStats(C.Structure):
@@ -83,6 +83,9 @@ _SIGS = {
     "apx_fcg": (_int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
     "apx_set_timing": (_int, [_vp, _int]),
     "apx_get_timing": (_int, [_vp, C.c_char_p, C.POINTER(_dbl), C.POINTER(_i64), C.POINTER(_i64)]),
+    "apx_set_timing_filter": (_int, [_vp, C.c_char_p]),
+    "apx_get_timer": (_int, [_vp, _int, C.c_char_p, _int, C.POINTER(_dbl), C.POINTER(_i64), C.POINTER(_i64),
+                             C.POINTER(_i64)]),
 }
 
 _lib = None

@@ -83,8 +83,9 @@ class DeviceProblem:
                 raise ValueError(f"model of set {i} expects {model.dofs_per_node(self.dim)} DOFs per node")
             conn = np.ascontiguousarray(_pick(conn_s, self.key), dtype=np.int64)
             kernel = model.kernel_for(conn.shape[1], self.dim) if hasattr(model, "kernel_for") else model.kernel
-            # user functions of x are evaluated at the set's distinct nodes (at most 64 of them)
-            self.samples.append(coords[np.unique(conn)[:64]])
+            # user functions of x ('user element' models) are checked for uniformity at every
+            # distinct node of the set
+            self.samples.append(coords[np.unique(conn)] if getattr(model, "user_element", False) else None)
             params = self._params(i, model, settings)
             sid = C.c_int()
             N.call("apx_plan_add_set", plan, conn.ctypes.data, conn.shape[0], conn.shape[1], kernel,
@@ -171,14 +172,35 @@ _CACHE = []
 _CACHE_MAX = 4
 
 
+def _digest(arrays):
+    """Content fingerprint of the arrays a plan is built from (shape, dtype and a 64-bit hash
+    of the bytes), so an array changed in place between solver calls builds a new plan
+    instead of reusing a stale one."""
+    try:
+        import xxhash
+        h = xxhash.xxh3_64()
+    except ImportError:  # pragma: no cover - xxhash ships with this image
+        import hashlib
+        h = hashlib.blake2b(digest_size=8)
+    for a in arrays:
+        a = np.ascontiguousarray(np.asarray(a))
+        h.update(repr((a.shape, a.dtype.str)).encode())
+        h.update(memoryview(a).cast("B"))
+    return h.hexdigest()
+
+
 def get_problem(dofs, settings, static_settings):
+    """The cached DeviceProblem of this mesh, mask and model tuple, or a new one.  The key holds
+    the identity of the coordinate, connectivity and mask objects and of the models, plus a
+    content fingerprint of the coordinates, connectivities and mask."""
     key_field = _field_key(dofs)
-    ident = (id(_pick(settings["node coordinates"], key_field)),
-             tuple(id(_pick(c, key_field)) for c in settings["connectivity"]),
-             tuple(id(v) for v in (settings["dirichlet dofs"].values() if isinstance(settings["dirichlet dofs"], dict)
-                                   else [settings["dirichlet dofs"]])),
+    coords = _pick(settings["node coordinates"], key_field)
+    conns = [_pick(c, key_field) for c in settings["connectivity"]]
+    masks = (list(settings["dirichlet dofs"].values()) if isinstance(settings["dirichlet dofs"], dict)
+             else [settings["dirichlet dofs"]])
+    ident = (id(coords), tuple(id(c) for c in conns), tuple(id(v) for v in masks),
              tuple(id(m) for m in static_settings["model"]),
-             np.shape(_pick(dofs, key_field)))
+             np.shape(_pick(dofs, key_field)), _digest([coords, *conns, *masks]))
     for entry in _CACHE:
         if entry[0] == ident:
             entry[1].update(settings)

@@ -259,8 +259,35 @@ int apx_newton(apx_plan* plan, double* u_dev, const double* dirichlet_values_dev
 int apx_set_timing(apx_plan* plan, int enabled) {
   return guarded([&] {
     APX_REQUIRE(plan, APX_ERR_ARG, "null plan");
-    plan->timing = enabled != 0;
+    APX_REQUIRE(enabled >= 0 && enabled <= 2, APX_ERR_ARG, "timing level must be 0, 1 or 2");
+    plan->timing = enabled;
     if (!enabled) plan->timers.clear();
+  });
+}
+
+int apx_set_timing_filter(apx_plan* plan, const char* names) {
+  return guarded([&] {
+    APX_REQUIRE(plan, APX_ERR_ARG, "null plan");
+    plan->timing_filter = names ? names : "";
+  });
+}
+
+int apx_get_timer(const apx_plan* plan, int index, char* name, int name_len, double* avg_ms, int64_t* count,
+                  int64_t* bytes, int64_t* flops) {
+  return guarded([&] {
+    APX_REQUIRE(plan, APX_ERR_ARG, "null plan");
+    APX_REQUIRE(index >= 0 && (size_t)index < plan->timers.size(), APX_ERR_ARG, "timer index out of range");
+    auto it = plan->timers.begin();
+    std::advance(it, index);
+    if (name && name_len > 0) {
+      std::strncpy(name, it->first.c_str(), (size_t)name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    const KernelTimer& t = it->second;
+    if (avg_ms) *avg_ms = t.count ? t.total_ms / (double)t.count : 0.0;
+    if (count) *count = t.count;
+    if (bytes) *bytes = t.bytes;
+    if (flops) *flops = t.flops;
   });
 }
 
@@ -474,6 +501,7 @@ int apx_fcg(apx_plan* plan, const void* vals_dev, const double* b_dev, double* x
     Plan& P = ready(plan);
     APX_REQUIRE(P.net, APX_ERR_STATE, "no network attached (apx_plan_attach_unet)");
     APX_REQUIRE(vals_dev && b_dev && x_dev, APX_ERR_ARG, "null device pointer");
+    APX_REQUIRE(precision == APX_FP64 || precision == APX_FP32, APX_ERR_ARG, "precision must be APX_FP64 or APX_FP32");
     void* sell = sell_buffer(P, precision);
     csr_to_sell(P, vals_dev, sell, precision, S(stream));
     FcgResult r = precision == APX_FP64
@@ -502,6 +530,8 @@ extern "C" int apx_unet_train_step(apx_unet* net, apx_plan* plan, const void* va
     APX_REQUIRE(vals_dev && r_list, APX_ERR_ARG, "null argument");
     void* sell = sell_buffer(P, APX_FP64);
     csr_to_sell(P, vals_dev, sell, APX_FP64, S(stream));
+    // the input scaling D^-1/2 r / s and the output map z = D^-1/2 s y use this matrix's diagonal
+    net_set_diag_scaling(P, sell, APX_FP64, S(stream));
     for (int b = 0; b < N.batch; ++b) {
       APX_REQUIRE(r_list[b], APX_ERR_ARG, "null residual in batch");
       net_input_from_reduced_img(P, N, r_list[b], 1, g_list ? g_list[b] : nullptr, nullptr, b, S(stream));

@@ -62,6 +62,8 @@ static inline int grid_chunks(int64_t n_chunks) {
   return (int)std::min<int64_t>(round_xcd(std::max<int64_t>(1, (n_chunks + wpb - 1) / wpb)), kMaxBlocks);
 }
 
+static_assert(kKrylovPartials == 4 * kMaxBlocks, "partials layout: 4 rows of kMaxBlocks");
+
 void ensure_krylov_scratch(Plan& P, hipStream_t st) {
   if (P.partials.n < (size_t)(4 * kMaxBlocks)) P.partials.alloc(4 * kMaxBlocks);
   if (P.kstate.n < 64) {
@@ -101,23 +103,25 @@ __device__ inline void finalize_sums(const double* partials, int nb, double (&su
   }
 }
 
-// Row gather of one SELL chunk: sum_k vals[k] * f(cols[k]) for this lane's row.
-template <class T, class F>
-__device__ __forceinline__ T sell_row(const T* __restrict__ v, const int32_t* __restrict__ ci, int w, F f) {
-  T s = 0;
+// Row gather of one SELL chunk: sum_k vals[k] * f(cols[k]) for this lane's row,
+// values of type M, accumulated in V (FP32 values with FP64 vectors: each
+// value is widened exactly and the products and sums are FP64).
+template <class M, class V = M, class F>
+__device__ __forceinline__ V sell_row(const M* __restrict__ v, const int32_t* __restrict__ ci, int w, F f) {
+  V s = 0;
   int k = 0;
   for (; k + 4 <= w; k += 4) {
     // values and columns are read once per SpMV: non-temporal, so L2 keeps the gathered vector
     const int32_t c0 = __builtin_nontemporal_load(ci + (k + 0) * 64), c1 = __builtin_nontemporal_load(ci + (k + 1) * 64),
                   c2 = __builtin_nontemporal_load(ci + (k + 2) * 64), c3 = __builtin_nontemporal_load(ci + (k + 3) * 64);
-    const T v0 = __builtin_nontemporal_load(v + (k + 0) * 64), v1 = __builtin_nontemporal_load(v + (k + 1) * 64),
+    const M v0 = __builtin_nontemporal_load(v + (k + 0) * 64), v1 = __builtin_nontemporal_load(v + (k + 1) * 64),
             v2 = __builtin_nontemporal_load(v + (k + 2) * 64), v3 = __builtin_nontemporal_load(v + (k + 3) * 64);
-    s += v0 * f(c0);
-    s += v1 * f(c1);
-    s += v2 * f(c2);
-    s += v3 * f(c3);
+    s += (V)v0 * f(c0);
+    s += (V)v1 * f(c1);
+    s += (V)v2 * f(c2);
+    s += (V)v3 * f(c3);
   }
-  for (; k < w; ++k) s += __builtin_nontemporal_load(v + k * 64) * f(__builtin_nontemporal_load(ci + k * 64));
+  for (; k < w; ++k) s += (V)__builtin_nontemporal_load(v + k * 64) * f(__builtin_nontemporal_load(ci + k * 64));
   return s;
 }
 
@@ -185,20 +189,47 @@ void csr_to_sell(Plan& P, const void* csr, void* sell, int precision, hipStream_
 }
 
 // ---- Jacobi ------------------------------------------------------------------------
-template <class T>
-__global__ void k_inv_diag(int64_t n, const int32_t* diag_slot, const T* vals, T* inv) {
+template <class M, class V = M>
+__global__ void k_inv_diag(int64_t n, const int32_t* diag_slot, const M* vals, const float* rowdef, V* inv) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    inv[i] = T(1) / vals[diag_slot[i]];
+    inv[i] = V(1) / ((V)vals[diag_slot[i]] + (rowdef ? (V)rowdef[i] : V(0)));
+}
+
+// Row-sum defect of a rounded SELL matrix, one lane per row (padding entries are zero in both).
+__global__ __launch_bounds__(512) void k_row_defect(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
+                                                    const double* __restrict__ v64, const float* __restrict__ v32,
+                                                    float* __restrict__ rowdef) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t c = blockIdx.x * 8 + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * 8) {
+    const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
+    double s = 0.0;
+    for (int k = 0; k < w; ++k) s += v64[off + k * 64 + lane] - (double)v32[off + k * 64 + lane];
+    const int64_t row = c * 64 + lane;
+    if (row < n) rowdef[row] = (float)s;
+  }
+}
+
+void row_defect(Plan& P, const double* v64, const float* v32, float* rowdef, hipStream_t st) {
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((P.n_chunks + 7) / 8, 65536));
+  hipLaunchKernelGGL(k_row_defect, dim3(g), dim3(512), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p, v64, v32, rowdef);
+  APX_LAUNCH_CHECK();
 }
 
 void jacobi_inverse(Plan& P, const void* vals, void* inv, int precision, hipStream_t st) {
   const int g = std::max(1, (int)std::min<int64_t>((P.n_free + 255) / 256, 65536));
   if (precision == APX_FP64)
     hipLaunchKernelGGL(k_inv_diag<double>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const double*)vals,
-                       (double*)inv);
+                       nullptr, (double*)inv);
   else
     hipLaunchKernelGGL(k_inv_diag<float>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const float*)vals,
-                       (float*)inv);
+                       nullptr, (float*)inv);
+  APX_LAUNCH_CHECK();
+}
+
+void jacobi_inverse_f32_to_f64(Plan& P, const float* vals, const float* rowdef, double* inv, hipStream_t st) {
+  const int g = std::max(1, (int)std::min<int64_t>((P.n_free + 255) / 256, 65536));
+  hipLaunchKernelGGL((k_inv_diag<float, double>), dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, vals, rowdef,
+                     inv);
   APX_LAUNCH_CHECK();
 }
 
@@ -259,11 +290,12 @@ __global__ __launch_bounds__(kFinBS) void k_pcg_init_fin(KState* S, const double
   }
 }
 
-template <class T>
+template <class M, class T>
 __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
-                                                      const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                      const int32_t* __restrict__ cols, const M* __restrict__ vals,
                                                       const T* __restrict__ z, T* __restrict__ pa, T* __restrict__ pb,
-                                                      T* __restrict__ q, KState* S, double* partials) {
+                                                      T* __restrict__ q, const float* __restrict__ rowdef, KState* S,
+                                                      double* partials) {
   if (halted(S)) return;
   constexpr int wpb = kSpmvBS / 64;
   const T beta = (T)S->beta;
@@ -274,14 +306,15 @@ __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunk
   double v[1] = {0.0};
   for (int64_t c = blockIdx.x * (int64_t)wpb + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * wpb) {
     const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
-    const T s = sell_row<T>(vals + off + lane, cols + off + lane, w,
-                            [&](int32_t col) { return z[col] + beta * po[col]; });
+    const T s = sell_row<M, T>(vals + off + lane, cols + off + lane, w,
+                               [&](int32_t col) { return z[col] + beta * po[col]; });
     const int64_t row = c * 64 + lane;
     if (row < n) {
       const T pi = z[row] + beta * po[row];
+      const T qi = rowdef ? s + (T)rowdef[row] * pi : s;
       pn[row] = pi;
-      q[row] = s;
-      v[0] += (double)pi * (double)s;
+      q[row] = qi;
+      v[0] += (double)pi * (double)qi;
     }
   }
   block_partials<kSpmvBS, 1>(v, partials);
@@ -375,8 +408,9 @@ __global__ __launch_bounds__(kFinBS) void k_pcg_reanchor_fin(KState* S, const do
 }
 
 // ---- PCG driver ----------------------------------------------------------------------
-template <class T>
-PcgRun<T>::PcgRun(Plan& P_, const T* vals_, const T* inv_, hipStream_t st_) : P(P_), vals(vals_), inv(inv_), st(st_) {
+template <class M, class T>
+PcgRun<M, T>::PcgRun(Plan& P_, const M* vals_, const T* inv_, hipStream_t st_)
+    : P(P_), vals(vals_), inv(inv_), st(st_) {
   n = P.n_free;
   T* w;
   if constexpr (sizeof(T) == 8) {
@@ -389,50 +423,61 @@ PcgRun<T>::PcgRun(Plan& P_, const T* vals_, const T* inv_, hipStream_t st_) : P(
   r = w; z = w + n; pa = w + 2 * n; pb = w + 3 * n; q = w + 4 * n; xw = w + 5 * n;
   ensure_krylov_scratch(P, st);
   S = (KState*)P.kstate.p;
+  part = P.partials.p;
   APX_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   APX_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
   // SELL values + columns, chunk offsets, z and p_old read, p_new and q written
-  spmv_bytes = P.sell_total * (int64_t)(sizeof(T) + 4) + (P.n_chunks + 1) * 4 + n * 4 * (int64_t)sizeof(T);
+  spmv_bytes = P.sell_total * (int64_t)(sizeof(M) + 4) + (P.n_chunks + 1) * 4 + n * 4 * (int64_t)sizeof(T);
   upd_bytes = n * 8 * (int64_t)sizeof(T);  // reads D^-1, p, q, x, r; writes x, r, z
 }
 
-template <class T>
-PcgRun<T>::~PcgRun() {
-  (void)hipEventDestroy(evs[0]);
-  (void)hipEventDestroy(evs[1]);
+template <class M, class T>
+PcgRun<M, T>::PcgRun(Plan& P_, const M* vals_, const T* inv_, hipStream_t st_, T* w, double* partials, KState* state)
+    : P(P_), vals(vals_), inv(inv_), st(st_), S(state), part(partials), external(true) {
+  n = P.n_free;
+  r = w; z = w + n; pa = w + 2 * n; pb = w + 3 * n; q = w + 4 * n; xw = w + 5 * n;
 }
 
-template <class T>
-void PcgRun<T>::init(const T* b, T* x_, double tol, double atol, int maxiter_, double delta) {
+template <class M, class T>
+PcgRun<M, T>::~PcgRun() {
+  if (evs[0]) (void)hipEventDestroy(evs[0]);
+  if (evs[1]) (void)hipEventDestroy(evs[1]);
+}
+
+template <class M, class T>
+void PcgRun<M, T>::init(const T* b, T* x_, double tol, double atol, int maxiter_, double delta) {
   x = x_ ? x_ : xw;
+  spmv_bytes = P.sell_total * (int64_t)(sizeof(M) + 4) + (P.n_chunks + 1) * 4 + n * 4 * (int64_t)sizeof(T) +
+               (sizeof(M) < sizeof(T) && rowdef ? n * 4 : 0);
   maxiter = maxiter_ <= 0 ? (int)std::min<int64_t>(10 * n, INT32_MAX / 2) : maxiter_;
   const int g = grid_for(n);
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(g), dim3(kBS), 0, st, n, b, inv, x, r, z, pa, S, P.partials.p, tol, atol,
+  hipLaunchKernelGGL(k_pcg_init<T>, dim3(g), dim3(kBS), 0, st, n, b, inv, x, r, z, pa, S, part, tol, atol,
                      maxiter, delta);
-  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, g, tol, atol, maxiter, delta);
+  hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(kFinBS), 0, st, S, part, g, tol, atol, maxiter, delta);
   APX_LAUNCH_CHECK();
 }
 
-template <class T>
-void PcgRun<T>::launch_iteration() {
+template <class M, class T>
+void PcgRun<M, T>::launch_iteration() {
   const int gs = grid_chunks(P.n_chunks), gu = grid_for(n);
-  EventPair e1 = timing_begin(P, st);
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals,
-                     z, pa, pb, q, S, P.partials.p);
-  timing_end(P, e1, st);
-  hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gs);
-  EventPair e2 = timing_begin(P, st);
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gu), dim3(kBS), 0, st, n, inv, pa, pb, q, x, r, z, S, P.partials.p);
-  timing_end(P, e2, st);
-  hipLaunchKernelGGL(k_pcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gu);
-  if (P.timing) {
+  const bool timed = P.timing && !external;
+  EventPair e1 = timed ? timing_begin(P, st) : EventPair{};
+  hipLaunchKernelGGL((k_pcg_spmv<M, T>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals,
+                     z, pa, pb, q, sizeof(M) < sizeof(T) ? rowdef : nullptr, S, part);
+  if (timed) timing_end(P, e1, st);
+  hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gs);
+  EventPair e2 = timed ? timing_begin(P, st) : EventPair{};
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gu), dim3(kBS), 0, st, n, inv, pa, pb, q, x, r, z, S, part);
+  if (timed) timing_end(P, e2, st);
+  hipLaunchKernelGGL(k_pcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gu);
+  if (timed) {
     ev_spmv.push_back(e1);
     ev_upd.push_back(e2);
   }
 }
 
-template <class T>
-KState PcgRun<T>::run() {
+template <class M, class T>
+KState PcgRun<M, T>::run() {
   constexpr int kBatch = 16;
   volatile int* flag = (volatile int*)P.pinned;  // [slot*4 + {k, maxiter, done, reliable}]
   auto enqueue_batch = [&](int slot) {
@@ -446,7 +491,18 @@ KState PcgRun<T>::run() {
   // flags after init / reanchor
   APX_HIP(hipMemcpyAsync((void*)(flag + 8), &S->k, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
   APX_HIP(hipStreamSynchronize(st));
-  if (!halted_host(2)) {
+  if (external && !halted_host(2)) {
+    // no events: enqueue a batch, copy the flags, wait for the stream, test
+    const int64_t limit = (int64_t)launched + maxiter + kBatch;
+    for (;;) {
+      for (int i = 0; i < kBatch; ++i) launch_iteration();
+      APX_LAUNCH_CHECK();
+      launched += kBatch;
+      APX_HIP(hipMemcpyAsync((void*)flag, &S->k, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+      APX_HIP(hipStreamSynchronize(st));
+      if (halted_host(0) || launched >= limit) break;
+    }
+  } else if (!halted_host(2)) {
     const int64_t limit = (int64_t)launched + maxiter + 2 * kBatch;
     enqueue_batch(0);
     int slot = 0;
@@ -457,35 +513,45 @@ KState PcgRun<T>::run() {
       slot ^= 1;
     }
   }
-  APX_HIP(hipStreamSynchronize(st));
   KState hs;
-  APX_HIP(hipMemcpy(&hs, S, sizeof(KState), hipMemcpyDeviceToHost));
+  if (external) {
+    // through the plan's pinned block (bytes 768..1023), so no pageable staging is involved
+    static_assert(sizeof(KState) <= 256, "KState must fit the pinned slot");
+    char* slot = (char*)P.pinned + 768;
+    APX_HIP(hipMemcpyAsync(slot, S, sizeof(KState), hipMemcpyDeviceToHost, st));
+    APX_HIP(hipStreamSynchronize(st));
+    std::memcpy(&hs, slot, sizeof(KState));
+  } else {
+    APX_HIP(hipStreamSynchronize(st));
+    APX_HIP(hipMemcpy(&hs, S, sizeof(KState), hipMemcpyDeviceToHost));
+  }
   iterations = hs.k;
   return hs;
 }
 
-template <class T>
-void PcgRun<T>::reanchor(const double* r_full) {
+template <class M, class T>
+void PcgRun<M, T>::reanchor(const double* r_full) {
   const int g = grid_for(n);
-  hipLaunchKernelGGL(k_pcg_reanchor<T>, dim3(g), dim3(kBS), 0, st, n, P.red2full.p, r_full, inv, x, r, z, S,
-                     P.partials.p);
-  hipLaunchKernelGGL(k_pcg_reanchor_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, g);
+  hipLaunchKernelGGL(k_pcg_reanchor<T>, dim3(g), dim3(kBS), 0, st, n, P.red2full.p, r_full, inv, x, r, z, S, part);
+  hipLaunchKernelGGL(k_pcg_reanchor_fin, dim3(1), dim3(kFinBS), 0, st, S, part, g);
   APX_LAUNCH_CHECK();
 }
 
-template <class T>
-void PcgRun<T>::collect_timing() {
-  if (!P.timing) return;
+template <class M, class T>
+void PcgRun<M, T>::collect_timing() {
+  if (!P.timing || external) return;
   const bool f64 = sizeof(T) == 8;
   // only launches that executed an iteration are counted; the rest exited at entry
   const size_t k = (size_t)iterations - counted;
-  timing_collect(P, f64 ? "pcg_spmv_f64" : "pcg_spmv_f32", ev_spmv, spmv_bytes, k);
+  timing_collect(P, sizeof(M) == sizeof(T) ? (f64 ? "pcg_spmv_f64" : "pcg_spmv_f32") : "pcg_spmv_f32m_f64v", ev_spmv,
+                 spmv_bytes, k);
   timing_collect(P, f64 ? "pcg_update_f64" : "pcg_update_f32", ev_upd, upd_bytes, k);
   counted = iterations;
 }
 
 template struct PcgRun<double>;
 template struct PcgRun<float>;
+template struct PcgRun<float, double>;
 
 template <class T>
 static apx_krylov_stats pcg_t(Plan& P, const T* vals, const T* inv, const T* b, T* x, double tol, double atol,

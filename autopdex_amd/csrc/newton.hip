@@ -14,12 +14,17 @@
 // (the residual norm) comes back to the host per Newton iteration.
 //
 // Mixed precision for models linear in u (reliable_delta > 0): the tangent
-// is assembled once (in FP64, rounded to FP32) and a single FP32 Jacobi-PCG runs across Newton
-// iterations; whenever its recursive residual has dropped by reliable_delta,
-// the FP32 correction is added to the FP64 DOFs, the FP64 residual is
-// assembled (the Newton test) and the CG residual is replaced by it while the
-// search direction is kept.  That is FP64 iterative refinement without
-// discarding the Krylov space at each refinement step.
+// is assembled once (in FP64, rounded to FP32) and a single Jacobi-PCG on the
+// FP32 matrix runs across Newton iterations; whenever its recursive residual
+// has dropped by reliable_delta, the correction is added to the FP64 DOFs, the
+// FP64 residual is assembled (the Newton test) and the CG residual is replaced
+// by it while the search direction is kept.  That is FP64 iterative
+// refinement without discarding the Krylov space at each refinement step.
+// The Krylov vectors are FP64 (krylov_fp64_vectors, each FP32 matrix value
+// widened in the SpMV) or FP32.  With FP64 vectors only the matrix is
+// perturbed (by its rounding, a relative 6e-8 per entry), so CG keeps the FP64
+// iteration count while the SpMV moves 4 bytes less per stored entry; FP32
+// vectors also lose orthogonality at FP32 rounding and need more iterations.
 #include <chrono>
 #include <cmath>
 
@@ -58,24 +63,43 @@ struct NewtonCtx {
   // rounded one (Jacobi-PCG: 20692 and 13456); one FP64 assembly plus the rounding cost far less
   // than one network application.
   bool fp32_arith = false;  // FP32 tangent: element arithmetic in FP32 instead of rounding the FP64 one
-  void tangent(void* vals, void* inv, int prec, int precond) {
+  // Row-sum defect of the rounded FP32 matrix (FP32 matrix with FP64 Krylov vectors, rounded
+  // from the FP64 tangent), or null.  Rounding every entry of the Poisson stencil to FP32 leaves
+  // each row sum off by ~1e-7 |A|, a diagonal perturbation of ~2 % of the smallest eigenvalue at
+  // 2048^2: CG on the rounded matrix drifts from the FP64 residual on smooth modes.  Adding the
+  // defect as a diagonal term (4 bytes per row in the SpMV) restores the FP64 row sums, so the
+  // remaining perturbation acts on rough modes only.
+  const float* rowdef = nullptr;
+  // Tangent values of type M into vals and the preconditioner's inverse diagonal of type V
+  // (an FP32 matrix may run with FP64 Krylov vectors: its diagonal is widened exactly).
+  template <class M, class V>
+  void tangent(M* vals, V* inv, int precond) {
+    const int prec = sizeof(M) == 8 ? APX_FP64 : APX_FP32;
     auto t0 = std::chrono::steady_clock::now();
     EventPair e = timing_begin(P, st);
+    rowdef = nullptr;
     if (prec == APX_FP32 && !fp32_arith) {
       if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
       assemble_tangent(P, u, P.vals64.p, APX_FP64, st);
       round_to_f32(P.vals64.p, (float*)vals, P.sell_total, st);
+      if constexpr (sizeof(M) < sizeof(V)) {
+        if (P.rowdef.n < (size_t)P.n_free) P.rowdef.alloc(P.n_free);
+        row_defect(P, P.vals64.p, (const float*)vals, P.rowdef.p, st);
+        rowdef = P.rowdef.p;
+      }
     } else {
       assemble_tangent(P, u, vals, prec, st);
     }
     timing_end(P, e, st);
     if (P.timing) ev_tan.push_back(e);
-    if (precond == APX_PRECOND_JACOBI)
-      jacobi_inverse(P, vals, inv, prec, st);
-    else if (prec == APX_FP64)
-      fill<double>((double*)inv, P.n_free, 1.0, st);
-    else
-      fill<float>((float*)inv, P.n_free, 1.0f, st);
+    if (precond == APX_PRECOND_JACOBI) {
+      if constexpr (sizeof(M) == sizeof(V))
+        jacobi_inverse(P, vals, inv, prec, st);
+      else
+        jacobi_inverse_f32_to_f64(P, vals, rowdef, inv, st);
+    } else {
+      fill<V>(inv, P.n_free, V(1), st);
+    }
     APX_HIP(hipStreamSynchronize(st));
     s.seconds_tangent += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -115,23 +139,30 @@ struct NewtonTest {
 };
 }  // namespace
 
-template <class T>
+// The plan's SELL value buffer of type M.
+template <class M>
+static M* matrix_buffer(Plan& P) {
+  if constexpr (sizeof(M) == 8) {
+    if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
+    return P.vals64.p;
+  } else {
+    if (P.vals32.n < (size_t)P.sell_total) P.vals32.alloc(P.sell_total);
+    return P.vals32.p;
+  }
+}
+
+// Newton with one Krylov solve per iteration.  M: tangent (matrix) type, V: Krylov vector type
+// of Jacobi-PCG (the learned path's flexible CG keeps FP64 vectors whatever M is).
+template <class M, class V>
 static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_options& o) {
   Plan& P = C.P;
   apx_newton_stats& s = C.s;
-  const int prec = sizeof(T) == 8 ? APX_FP64 : APX_FP32;
-  DevBuf<T> vecs;
+  const int vprec = sizeof(V) == 8 ? APX_FP64 : APX_FP32;
+  DevBuf<V> vecs;
   vecs.alloc(2 * P.n_free);
-  T* inv = vecs.p;
-  T* b = vecs.p + P.n_free;
-  T* vals;
-  if constexpr (sizeof(T) == 8) {
-    if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
-    vals = P.vals64.p;
-  } else {
-    if (P.vals32.n < (size_t)P.sell_total) P.vals32.alloc(P.sell_total);
-    vals = P.vals32.p;
-  }
+  V* inv = vecs.p;
+  V* b = vecs.p + P.n_free;
+  M* vals = matrix_buffer<M>(P);
   impose_dirichlet(P, C.u, dvals, C.st);
   C.residual();
   const double res0 = C.norm();
@@ -142,7 +173,7 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
   double res = res0;
   bool have_tangent = false, go = true;
   bool learned = o.precond == APX_PRECOND_LEARNED, fell_back = false;
-  PcgRun<T> pcg(P, vals, inv, C.st);
+  PcgRun<M, V> pcg(P, vals, inv, C.st);
   DevBuf<double> b64, x64;
   if (learned) {
     APX_REQUIRE(P.net != nullptr, APX_ERR_STATE, "learned preconditioner requested but no network is attached");
@@ -151,7 +182,8 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
   }
   while (go) {
     if (!have_tangent || !o.reuse_tangent) {
-      C.tangent(vals, inv, prec, learned ? APX_PRECOND_JACOBI : o.precond);
+      C.tangent(vals, inv, learned ? APX_PRECOND_JACOBI : o.precond);
+      pcg.rowdef = C.rowdef;
       have_tangent = true;
     }
     double eta = o.lin_tol;
@@ -160,11 +192,11 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
     // stop each FP32 flexible-CG solve after a 'reliable delta' reduction and let the FP64
     // Newton residual re-anchor the next one (flexible CG(1) keeps one direction, so the
     // restart costs little)
-    if (learned && prec == APX_FP32 && o.reliable_delta > 0.0) eta = std::max(eta, o.reliable_delta);
+    if (learned && sizeof(M) == 4 && o.reliable_delta > 0.0) eta = std::max(eta, o.reliable_delta);
     auto t0 = std::chrono::steady_clock::now();
     if (learned) {
       gather_rhs(P, C.r, b64.p, APX_FP64, C.st);
-      FcgResult fr = fcg_solve<T>(P, *P.net, vals, P.net_g, b64.p, x64.p, eta, o.lin_atol, o.lin_maxiter, C.st);
+      FcgResult fr = fcg_solve<M>(P, *P.net, vals, P.net_g, b64.p, x64.p, eta, o.lin_atol, o.lin_maxiter, C.st);
       s.krylov_iterations += fr.iterations;
       if (fr.stagnated) {
         // the network stopped reducing the residual: keep its progress, continue with Jacobi-PCG
@@ -174,7 +206,7 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
       }
       scatter_update(P, C.u, x64.p, o.damping, APX_FP64, C.st);
     } else {
-      gather_rhs(P, C.r, b, prec, C.st);
+      gather_rhs(P, C.r, b, vprec, C.st);
       // after a fallback the iteration budget is Jacobi-PCG's own (at least jax cg's default
       // 10 n_free): 'linear maxiter' was sized for flexible CG with the network
       const int fb_max = (int)std::min<int64_t>(10 * P.n_free, 2000000000);
@@ -185,7 +217,7 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
       KState hs = pcg.run();
       pcg.collect_timing();
       s.krylov_iterations += hs.k;
-      scatter_update(P, C.u, pcg.x, o.damping, prec, C.st);
+      scatter_update(P, C.u, pcg.x, o.damping, vprec, C.st);
     }
     s.seconds_krylov += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     C.residual();
@@ -197,24 +229,27 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
   s.residual_norm = res;
 }
 
+// V: Krylov vector type (FP32, or FP64 with the FP32 matrix widened in the SpMV).
+template <class V>
 static void newton_reliable(NewtonCtx& C, const double* dvals, const apx_newton_options& o) {
   Plan& P = C.P;
   apx_newton_stats& s = C.s;
-  DevBuf<float> vecs;
+  const int vprec = sizeof(V) == 8 ? APX_FP64 : APX_FP32;
+  DevBuf<V> vecs;
   vecs.alloc(2 * P.n_free);
-  float* inv = vecs.p;
-  float* b = vecs.p + P.n_free;
-  if (P.vals32.n < (size_t)P.sell_total) P.vals32.alloc(P.sell_total);
-  float* vals = P.vals32.p;
+  V* inv = vecs.p;
+  V* b = vecs.p + P.n_free;
+  float* vals = matrix_buffer<float>(P);
   impose_dirichlet(P, C.u, dvals, C.st);
   C.residual();
   const double res0 = C.norm();
   s.initial_residual = res0;
   const double thr = o.relative ? o.newton_tol * res0 : o.newton_tol;
-  C.tangent(vals, inv, APX_FP32, o.precond);
-  gather_rhs(P, C.r, b, APX_FP32, C.st);
+  C.tangent(vals, inv, o.precond);
+  gather_rhs(P, C.r, b, vprec, C.st);
   auto t0 = std::chrono::steady_clock::now();
-  PcgRun<float> pcg(P, vals, inv, C.st);
+  PcgRun<float, V> pcg(P, vals, inv, C.st);
+  pcg.rowdef = C.rowdef;
   // the FP32 recursion stops on its own at half the Newton threshold; the
   // FP64 residual decides at every anchor
   pcg.init(b, nullptr, 0.0, 0.5 * thr, o.lin_maxiter, o.reliable_delta);
@@ -224,7 +259,7 @@ static void newton_reliable(NewtonCtx& C, const double* dvals, const apx_newton_
   while (go) {
     KState hs = pcg.run();
     pcg.collect_timing();
-    scatter_update(P, C.u, pcg.x, 1.0, APX_FP32, C.st);
+    scatter_update(P, C.u, pcg.x, 1.0, vprec, C.st);
     C.residual();
     res = C.norm();
     go = test.step(res, thr, o.maxiter, s);
@@ -250,13 +285,20 @@ void newton(Plan& P, double* u, const double* dvals, const apx_newton_options& o
   s = apx_newton_stats{};
   NewtonCtx C{P, u, P.r_full.p, st, {}, {}, s};
   C.fp32_arith = o.fp32_element_arith != 0;
+  APX_REQUIRE(o.precision == APX_FP64 || o.precision == APX_FP32, APX_ERR_ARG, "apx_newton: unknown precision");
+  const bool v64 = o.krylov_fp64_vectors != 0;
   if (o.precision == APX_FP32 && o.reuse_tangent && o.reliable_delta > 0.0 && o.precond != APX_PRECOND_LEARNED) {
     APX_REQUIRE(o.damping == 1.0, APX_ERR_ARG, "reliable-update refinement needs damping 1");
-    newton_reliable(C, dvals, o);
+    if (v64)
+      newton_reliable<double>(C, dvals, o);
+    else
+      newton_reliable<float>(C, dvals, o);
   } else if (o.precision == APX_FP64) {
-    newton_standard<double>(C, dvals, o);
+    newton_standard<double, double>(C, dvals, o);
+  } else if (v64) {
+    newton_standard<float, double>(C, dvals, o);
   } else {
-    newton_standard<float>(C, dvals, o);
+    newton_standard<float, float>(C, dvals, o);
   }
   C.finish(o.precision);
   s.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -303,7 +303,8 @@ void plan_finalize(Plan& P) {
     P.sets[si].conn.upload(conn_perm[si].data(), conn_perm[si].size());
     P.sets[si].slots.upload(slots_perm[si].data(), slots_perm[si].size());
   }
-  // pinned host scalars: [0, 512) B PCG flags and norms, [512, 768) B FCG state copies
+  // pinned host scalars: [0, 512) B PCG flags and norms, [512, 768) B FCG state copies,
+  // [768, 1024) B the final Krylov state of an allocation-free PCG run
   if (!P.pinned) APX_HIP(hipHostMalloc((void**)&P.pinned, 128 * sizeof(double), hipHostMallocDefault));
   if (!P.ev0) APX_HIP(hipEventCreate(&P.ev0));
   if (!P.ev1) APX_HIP(hipEventCreate(&P.ev1));

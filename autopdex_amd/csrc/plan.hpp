@@ -39,7 +39,8 @@ struct ElemSet {
 struct KernelTimer {
   double total_ms = 0.0;
   int64_t count = 0;
-  int64_t bytes = 0;  // algorithmic bytes per launch
+  int64_t bytes = 0;  // algorithmic bytes per launch (the average over the launches timed)
+  int64_t flops = 0;  // algorithmic flops per launch (matrix-core work; 0 for streaming kernels)
 };
 
 struct Plan {
@@ -77,6 +78,9 @@ struct Plan {
   // workspace (lazily sized)
   DevBuf<double> vals64, w64;  // nnz, vectors
   DevBuf<float> vals32, w32;
+  // row-sum defect of the FP32 matrix rounded from the FP64 one: rowdef_i = sum_j (A64_ij -
+  // A32_ij), n_free (the diagonal correction of the FP32-matrix SpMV with FP64 vectors)
+  DevBuf<float> rowdef;
   DevBuf<double> r_full, diag_full;  // n_dofs
   DevBuf<double> partials;     // reduction partials
   DevBuf<double> tw, tpart;    // network training workspace (4 n_free) and its reduction partials
@@ -89,7 +93,11 @@ struct Plan {
   // Set from the matrix of the solve or training step that uses the network.
   DevBuf<double> net_dsq;
   bool net_dsq_set = false;
-  bool timing = false;
+  // 0 off; 1 kernel-family timers of the solvers, plus one timer per launched network kernel
+  // (name = its template, e.g. "k_conv3x3<64, 64, false, 4>") for the names in timing_filter;
+  // 2 every network kernel
+  int timing = 0;
+  std::string timing_filter;  // comma-separated kernel names timed per launch at level 1
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> evpool;  // recycled timing events

@@ -141,6 +141,19 @@ void launch_wgrad(int ci, int co, const WgradArgs& a, hipStream_t st) {
                                        std::to_string(co) + " channels");
 }
 
+// The template name of the kernel launch_conv runs for these arguments (the names rocprofv3
+// shows, without namespace and argument list), for per-launch timers.
+std::string conv_kernel_name(int ci, int co, bool f32, const ConvArgs& a) {
+  auto s = [](int v) { return std::to_string(v); };
+  if (f32) return "k_conv_out<" + s(co) + ">";
+  if ((ci == 16 || ci == 32) && (co == 16 || co == 32)) return "k_conv3x3_ws<" + s(ci) + ", " + s(co) + ", 4, 0>";
+  if ((ci == 1 || ci == 2) && co == 16 && !a.xmask) return "k_conv_in<" + s(ci) + ">";
+  const int nt = round16(co) / 16;
+  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
+  const int nb = (nt > 1 && tiles < kSplitTiles) ? 1 : nt;
+  return "k_conv3x3<" + s(ci) + ", " + s(co) + ", false, " + s(nb) + ">";
+}
+
 // ---- grid transfer ---------------------------------------------------------------------
 // P: fine(i, j) = sum_{I,J} w(i - 2I) w(j - 2J) coarse(I, J), w(0) = 1, w(+-1) = 1/2.
 // out = skip + P(coarse) (skip may alias out).  One thread per (pixel, 8 channels):
@@ -889,51 +902,125 @@ void UNet::collect_timing(size_t count) {
   if (!timer_plan) return;
   timing_collect(*timer_plan, "unet_conv_d0", ev_conv0, levels > 1 ? conv_bytes(L_d[0], 0) : 0, count);
   timing_collect(*timer_plan, "unet_forward", ev_fwd, forward_bytes(), count);
+  // per-launch kernel records of the first `count` forward passes
+  Plan& P = *timer_plan;
+  for (auto& kv : ev_launch) {
+    auto& t = P.timers[kv.first];
+    double ms_sum = t.total_ms;
+    int64_t n0 = t.count, bytes_sum = t.bytes * t.count, flops_sum = t.flops * t.count;
+    for (auto& rec : kv.second) {
+      APX_HIP(hipEventSynchronize(rec.e.b));
+      if (rec.fwd < count) {
+        float ms = 0.f;
+        APX_HIP(hipEventElapsedTime(&ms, rec.e.a, rec.e.b));
+        ms_sum += ms;
+        bytes_sum += rec.bytes;
+        flops_sum += rec.flops;
+        ++n0;
+      }
+      P.evpool.push_back(rec.e.a);
+      P.evpool.push_back(rec.e.b);
+    }
+    t.total_ms = ms_sum;
+    t.count = n0;
+    t.bytes = n0 ? bytes_sum / n0 : 0;
+    t.flops = n0 ? flops_sum / n0 : 0;
+    kv.second.clear();
+  }
+  n_fwd = 0;
 }
+
+namespace {
+bool filter_has(const std::string& filter, const std::string& name) {
+  size_t pos = 0;
+  while (pos <= filter.size()) {
+    const size_t end = std::min(filter.find(',', pos), filter.size());
+    if (filter.compare(pos, end - pos, name) == 0) return true;
+    pos = end + 1;
+  }
+  return false;
+}
+}  // namespace
 
 void UNet::forward(hipStream_t st, bool keep_activations) {
   APX_REQUIRE(weights_ready, APX_ERR_STATE, "network weights not prepared");
   const int L = levels;
   const bool timed = timer_plan && timer_plan->timing;
   EventPair ef = timed ? timing_begin(*timer_plan, st) : EventPair{};
+  // per-launch records: every kernel at timing level 2, the filtered names at level 1
+  const bool rec_all = timed && timer_plan->timing >= 2;
+  const bool rec_some = timed && !timer_plan->timing_filter.empty();
+  const size_t fwd_index = n_fwd;
+  auto record = [&](const std::string& name, int64_t bytes, int64_t flops, auto&& launch) {
+    if (!(rec_all || (rec_some && filter_has(timer_plan->timing_filter, name)))) return launch();
+    EventPair e = timing_begin(*timer_plan, st);
+    launch();
+    timing_end(*timer_plan, e, st);
+    ev_launch[name].push_back(LaunchRec{e, fwd_index, bytes, flops});
+  };
+  auto conv = [&](int li, int level, const uint16_t* x, void* y) {
+    if (!(rec_all || rec_some)) return conv_fwd(*this, li, level, x, y, st);
+    const NetLayer& ly = layers[li];
+    ConvArgs probe{};
+    probe.B = batch;
+    probe.H = H[level];
+    probe.W = W[level];
+    record(conv_kernel_name(ly.ci, ly.co, ly.out_f32 != 0, probe), conv_bytes(li, level),
+           (int64_t)18 * ly.ci * ly.co * pixels(level), [&] { conv_fwd(*this, li, level, x, y, st); });
+  };
+  auto transfer = [&](bool restriction, int l) {
+    const int C8 = ch[l] / 8;
+    const int64_t bytes = (restriction ? pixels(l) : 2 * pixels(l)) * ch[l] * 2 + pixels(l + 1) * ch[l] * 2;
+    const std::string name = std::string(restriction ? "k_restrict<" : "k_prolong_add<") + std::to_string(C8) + ">";
+    if (restriction)
+      record(name, bytes, 0, [&] { restrict_(*this, l, ch[l], d[l].p, rd[l].p, st); });
+    else
+      record(name, bytes, 0, [&] { prolong_add(*this, l, ch[l], t[l].p, d[l].p, s[l].p, st); });
+  };
   // levels >= c from the restriction of level c to its prolongation: one launch (the tail)
   const bool fused = fuse_tail && tail_level >= 0;
   const int c = fused ? tail_level : L;
   // inference: the input layer is formed inside the level-0 conv_e launch (its output a_0 is
   // read by nothing else; training keeps it for the backward pass)
   const bool fused_in = fuse_tail && !keep_activations && L > 1 && ch[0] == 16;
-  if (fused_in)
-    conv_fwd_in_e0(*this, st);
-  else
-    conv_fwd(*this, L_in, 0, x0.p, a[0].p, st);
+  if (fused_in) {
+    const NetLayer& le = layers[L_e[0]];
+    record("k_conv3x3_ws<16, 16, 4, " + std::to_string(c_in) + ">",
+           pixels(0) * (c_in * 2 + le.co * 2) + (int64_t)(layers[L_in].cop * layers[L_in].wk + le.cop * le.wk) * 2,
+           (int64_t)18 * (c_in * 16 + 16 * le.co) * pixels(0), [&] { conv_fwd_in_e0(*this, st); });
+  } else {
+    conv(L_in, 0, x0.p, a[0].p);
+  }
   for (int l = 0; l + 1 < L && l <= c; ++l) {
-    if (l > 0 || !fused_in) conv_fwd(*this, L_e[l], l, a[l].p, d[l].p, st);
+    if (l > 0 || !fused_in) conv(L_e[l], l, a[l].p, d[l].p);
     if (l == c) break;
-    restrict_(*this, l, ch[l], d[l].p, rd[l].p, st);
-    conv_fwd(*this, L_down[l], l + 1, rd[l].p, a[l + 1].p, st);
+    transfer(true, l);
+    conv(L_down[l], l + 1, rd[l].p, a[l + 1].p);
   }
   if (fused)
-    hipLaunchKernelGGL(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop);
+    record("k_unet_tail", 0, 0,
+           [&] { hipLaunchKernelGGL(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop); });
   else
-    conv_fwd(*this, L_b, L - 1, a[L - 1].p, u[L - 1].p, st);
+    conv(L_b, L - 1, a[L - 1].p, u[L - 1].p);
   for (int l = std::min(L - 2, c); l >= 0; --l) {
     if (l != c) {
-      conv_fwd(*this, L_up[l], l + 1, u[l + 1].p, t[l].p, st);
-      prolong_add(*this, l, ch[l], t[l].p, d[l].p, s[l].p, st);
+      conv(L_up[l], l + 1, u[l + 1].p, t[l].p);
+      transfer(false, l);
     }
     EventPair ec{};
     if (timed && l == 0) ec = timing_begin(*timer_plan, st);
-    conv_fwd(*this, L_d[l], l, s[l].p, u[l].p, st);
+    conv(L_d[l], l, s[l].p, u[l].p);
     if (timed && l == 0) {
       timing_end(*timer_plan, ec, st);
       ev_conv0.push_back(ec);
     }
   }
-  conv_fwd(*this, L_out, 0, u[0].p, y.p, st);
+  conv(L_out, 0, u[0].p, y.p);
   APX_LAUNCH_CHECK();
   if (timed) {
     timing_end(*timer_plan, ef, st);
     ev_fwd.push_back(ef);
+    ++n_fwd;
   }
 }
 

@@ -67,6 +67,15 @@ struct UNet {
   // launch timing (HIP events on the launch stream) when the attached plan times its kernels
   struct Plan* timer_plan = nullptr;
   std::vector<EventPair> ev_conv0, ev_fwd;  // level-0 conv_d launches, forward passes
+  // per-launch records of individual kernels (timing level 2, or the plan's filter): the
+  // forward pass they belong to, algorithmic bytes and flops
+  struct LaunchRec {
+    EventPair e;
+    size_t fwd;
+    int64_t bytes, flops;
+  };
+  std::map<std::string, std::vector<LaunchRec>> ev_launch;
+  size_t n_fwd = 0;  // forward passes recorded since the last collect
   // optional device stop flag (a Krylov solver's done flag): forward-pass kernels launched
   // after it is set return at once
   const int* stop = nullptr;

@@ -213,16 +213,24 @@ def line_traction_potential(traction, multiplier_key="load multiplier"):
 
 def _uniform(fun, sample_x, settings, what):
     """fun(x) or fun(x, settings) (the reference dispatches on the parameter count,
-    models.py:937-944) at the sample points; it must not vary over them."""
-    if sample_x is None:
+    models.py:937-944) at every point of sample_x (every distinct node of the element set);
+    it must not vary over them, because the device kernels take it uniform.  A function that
+    varies raises NotImplementedError."""
+    if sample_x is None or len(sample_x) == 0:
         raise ValueError(f"{what}: sample points are needed to evaluate the user function")
     two = len(signature(fun).parameters) != 1
-    vals = [np.asarray(fun(np.asarray(x), settings) if two else fun(np.asarray(x))) for x in sample_x]
-    vals = [v.astype(np.complex128 if np.iscomplexobj(v) else np.float64) for v in vals]
-    for v in vals[1:]:
-        if v.shape != vals[0].shape or not np.allclose(v, vals[0], rtol=1e-14, atol=0.0):
-            raise NotImplementedError(f"{what} varies over the element set; the device kernels take it uniform")
-    return vals[0]
+
+    def at(x):
+        v = np.asarray(fun(np.asarray(x), settings) if two else fun(np.asarray(x)))
+        return v.astype(np.complex128 if np.iscomplexobj(v) else np.float64)
+
+    first = at(sample_x[0])
+    for x in sample_x[1:]:
+        v = at(x)
+        if v.shape != first.shape or not np.allclose(v, first, rtol=1e-14, atol=0.0):
+            raise NotImplementedError(f"{what} varies over the element set (at x = {np.asarray(x).tolist()}); "
+                                      "the device kernels take it uniform")
+    return first
 
 
 class _HyperelasticWeak:

@@ -12,8 +12,9 @@ static_settings keys read here (reference keys unchanged):
   'verbose'                as in the reference
 new keys of this backend:
   'hip precision'          'fp64' (default): FP64 tangent + FP64 PCG;
-                           'mixed': FP32 tangent + FP32 PCG inner solves, the
-                           Newton residual stays FP64 (iterative refinement)
+                           'mixed': FP32 tangent (Krylov matrix) for the inner
+                           solves, the Newton residual and its norm stay FP64
+                           (iterative refinement)
   'linear tolerance'       Krylov relative tolerance (default 1e-5, the
                            default of jax.scipy.sparse.linalg.cg that the
                            reference's newton mode uses, P7)
@@ -23,10 +24,16 @@ new keys of this backend:
   'linear forcing'         True: Krylov tolerance max(linear tolerance,
                            0.5 tol/||r_k||) -- no inner solve tighter than the
                            Newton test needs (default False: fixed tolerance)
-  'reliable delta'         mixed precision on models linear in u: one FP32 CG
-                           re-anchored to the FP64 residual each time its
-                           residual drops by this factor (default 0.1; 0 turns
-                           it off and restarts CG at every Newton step)
+  'reliable delta'         mixed precision on models linear in u: one CG on the
+                           FP32 matrix re-anchored to the FP64 residual each
+                           time its residual drops by this factor (default 0.1;
+                           0 turns it off and restarts CG at every Newton step)
+  'hip tangent arithmetic' 'fp64' (default: in 'mixed' the FP64 tangent is
+                           rounded once to FP32) | 'fp32' (element arithmetic of
+                           the tangent in FP32)
+  'hip krylov vectors'     'fp64' (default: with 'mixed', Jacobi-PCG keeps FP64
+                           vectors and widens the FP32 matrix in the SpMV) |
+                           'fp32' (FP32 vectors: an all-FP32 inner solve)
 Newton semantics (P4-P6) are implemented in the device driver apx_newton
 (solver.py:740-948).
 """
@@ -98,6 +105,10 @@ def newton_options(static_settings, newton_tol=1e-8, maxiter=30, damping=1.0, re
     if arith not in ("fp64", "fp32"):
         raise ValueError("'hip tangent arithmetic' must be 'fp64' (FP64 tangent rounded to FP32 in 'mixed') or 'fp32'")
     o.fp32_element_arith = 1 if arith == "fp32" else 0
+    vecs = _opt(static_settings, "hip krylov vectors", "fp64")
+    if vecs not in ("fp64", "fp32"):
+        raise ValueError("'hip krylov vectors' must be 'fp64' (FP32 matrix with FP64 Jacobi-PCG vectors) or 'fp32'")
+    o.krylov_fp64_vectors = 1 if vecs == "fp64" else 0
     return o
 
 

@@ -175,6 +175,10 @@ typedef struct apx_newton_options {
   int fp32_element_arith;   /* precision FP32 only.  0: the tangent is assembled in FP64
                                and rounded once to FP32 (default); 1: the element
                                arithmetic of the tangent runs in FP32 (FP32 accumulation)  */
+  int krylov_fp64_vectors;  /* precision FP32 only.  1: Jacobi-PCG keeps FP64 vectors and
+                               scalars with the FP32 matrix (each value widened in the
+                               SpMV); 0: FP32 vectors.  Flexible CG always uses FP64
+                               vectors.                                                    */
 } apx_newton_options;
 
 typedef struct apx_newton_stats {
@@ -274,9 +278,20 @@ const char* apx_last_error(void);
 int apx_version(void);
 /* Average device time in ms of the most recent launches of a named kernel
  * family ("spmv", "residual", "tangent", "pcg_update") recorded with HIP
- * events on the caller's stream (0 when never timed).                       */
+ * events on the caller's stream (0 when never timed).  enabled: 0 off (clears
+ * the timers), 1 the solvers' kernel families plus, per launch, the network
+ * kernels named in the filter, 2 also every network kernel per launch (timer
+ * name = the kernel's template as rocprofv3 shows it, e.g.
+ * "k_conv3x3<64, 64, false, 4>").  No counterpart in the reference
+ * (measurement of this implementation).                                      */
 int apx_set_timing(apx_plan* plan, int enabled);
 int apx_get_timing(apx_plan* plan, const char* name, double* avg_ms, int64_t* count, int64_t* bytes);
+/* Comma-separated kernel names timed per launch at timing level 1.          */
+int apx_set_timing_filter(apx_plan* plan, const char* names);
+/* Timer `index` (0-based, name order): name, average ms per launch, launches,
+ * algorithmic bytes and flops per launch; APX_ERR_ARG past the last timer.   */
+int apx_get_timer(const apx_plan* plan, int index, char* name, int name_len, double* avg_ms, int64_t* count,
+                  int64_t* bytes, int64_t* flops);
 
 #ifdef __cplusplus
 }

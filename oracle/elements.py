@@ -104,12 +104,44 @@ def _gauss(dim, npe=None):
     return gauss_legendre_nd(dim, 4 if npe in (3, 9) else 2)
 
 
+def mat_det(m):
+    """utility.matrix_det (utility.py:771-817): closed-form determinant of (..., n, n), n = 1, 2, 3."""
+    n = m.shape[-1]
+    if n == 1:
+        return m[..., 0, 0]
+    if n == 2:
+        return m[..., 0, 0] * m[..., 1, 1] - m[..., 0, 1] * m[..., 1, 0]
+    a, b, c = m[..., 0, 0], m[..., 0, 1], m[..., 0, 2]
+    d, e, f = m[..., 1, 0], m[..., 1, 1], m[..., 1, 2]
+    g, h, i = m[..., 2, 0], m[..., 2, 1], m[..., 2, 2]
+    return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g)
+
+
+def mat_inv(m):
+    """utility.matrix_inv (utility.py:705-769): adjugate / determinant of (..., n, n), n = 1, 2, 3."""
+    n = m.shape[-1]
+    if n == 1:
+        return 1.0 / m
+    if n == 2:
+        a, b, c, d = m[..., 0, 0], m[..., 0, 1], m[..., 1, 0], m[..., 1, 1]
+        inv_det = 1.0 / (a * d - b * c)
+        return inv_det[..., None, None] * np.stack([np.stack([d, -b], -1), np.stack([-c, a], -1)], -2)
+    a, b, c = m[..., 0, 0], m[..., 0, 1], m[..., 0, 2]
+    d, e, f = m[..., 1, 0], m[..., 1, 1], m[..., 1, 2]
+    g, h, i = m[..., 2, 0], m[..., 2, 1], m[..., 2, 2]
+    inv_det = 1.0 / mat_det(m)
+    adj = np.stack([np.stack([e * i - f * h, c * h - b * i, b * f - c * e], -1),
+                    np.stack([f * g - d * i, a * i - c * g, c * d - a * f], -1),
+                    np.stack([d * h - e * g, b * g - a * h, a * e - b * d], -1)], -2)
+    return inv_det[..., None, None] * adj
+
+
 def geometry(xe, dN):
     """Per element: J_ij = sum_a x_ai dN_aj, signed det J, G = dN J^-1 (shape (E, npe, dim))."""
-    J = np.einsum("eai,aj->eij", xe, dN)
-    det = np.linalg.det(J)
-    Jinv = np.linalg.inv(J)
-    G = np.einsum("aj,ejk->eak", dN, Jinv)
+    J = np.swapaxes(xe, 1, 2) @ dN
+    det = mat_det(J)
+    Jinv = mat_inv(J)
+    G = dN[None] @ Jinv
     return det, G
 
 
@@ -136,8 +168,8 @@ class PoissonSource:
         for q in range(len(wq)):
             N, dN = shape(npe, dim, xq[q])
             det, G = geometry(xe, dN)
-            x = np.einsum("a,eai->ei", N, xe)
-            grad = np.einsum("ea,eai->ei", ue, G)
+            x = N @ xe
+            grad = (ue[:, None, :] @ G)[:, 0, :]
             yield wq[q] * det, N, G, x, grad
 
     def potential(self, xe, ue, nodal=None):
@@ -150,7 +182,7 @@ class PoissonSource:
     def residual(self, xe, ue, nodal=None):
         out = np.zeros(ue.shape)
         for wd, N, G, x, grad in self._loop(xe, ue):
-            out += wd[:, None] * (np.einsum("eai,ei->ea", G, grad) - self.source(x)[:, None] * N[None, :])
+            out += wd[:, None] * ((G @ grad[:, :, None])[:, :, 0] - self.source(x)[:, None] * N[None, :])
         return out
 
     def tangent(self, xe, ue, nodal=None):

@@ -88,6 +88,23 @@ def linear_solve(A, b, kind="lapack", tol=1e-5, atol=0.0, maxiter=None):
     raise ValueError(kind)
 
 
+def solve_linear(dofs, residual_fn, tangent_fn, dirichlet_mask, dirichlet_values, lin_solve):
+    """solve_linear (solver.py:544-659) on flat DOFs with nodal imposition: the Dirichlet entries
+    of dofs are overwritten with their prescribed values (:584-604), the residual and reduced
+    tangent are assembled there, A x = -r[free] is solved (P4), and the result holds the
+    Dirichlet *values* at Dirichlet DOFs and the *increment* x at free DOFs (:646-656)."""
+    dmask = np.asarray(dirichlet_mask, dtype=bool).ravel()
+    free = ~dmask
+    dvals = np.asarray(dirichlet_values, dtype=np.float64).ravel()
+    imp = np.asarray(dofs, dtype=np.float64).ravel().copy()
+    imp[dmask] = dvals[dmask]
+    rhs = -residual_fn(imp)
+    A = tangent_fn(imp)
+    delta = imp.copy()
+    delta[free] = lin_solve(A, rhs[free])
+    return delta
+
+
 def newton(u0, residual_fn, tangent_fn, dirichlet_mask, dirichlet_values, lin_solve,
            tol=1e-8, maxiter=30, damping=1.0, history=None):
     """damped_newton (solver.py:837-948) around solve_linear (:544-659) on flat DOFs.
@@ -102,13 +119,7 @@ def newton(u0, residual_fn, tangent_fn, dirichlet_mask, dirichlet_values, lin_so
     itt, res_old, not_stop, div, res = 0, 0.0, True, False, 0.0
     while not_stop:
         # solve_linear: impose Dirichlet values, assemble, solve, compose (P5)
-        imp = dofs.copy()
-        imp[dmask] = dvals[dmask]
-        rhs = -residual_fn(imp)
-        A = tangent_fn(imp)
-        x = lin_solve(A, rhs[free])
-        delta = imp.copy()
-        delta[free] = x
+        delta = solve_linear(dofs, residual_fn, tangent_fn, dmask, dvals, lin_solve)
         # damped update
         delta[free] *= damping
         dofs[free] += delta[free]

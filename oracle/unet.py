@@ -170,13 +170,19 @@ class UNetOracle:
 
     def loss_backward(self, target):
         """loss = mean_b ||y_b - t_b||^2 / ||t_b||^2; returns (loss, grad) with the device layout."""
-        q, L, A, theta = self.q, self.L, self.A, self.theta_used
-        y = A["y"]
+        y = self.A["y"]
         B = y.shape[0]
         t2 = (target ** 2).reshape(B, -1).sum(1)
         e2 = ((y - target) ** 2).reshape(B, -1).sum(1)
         loss = float(np.mean(e2 / t2))
-        gy = q(2 * (y - target) / (t2[:, None, None, None] * B))
+        return loss, self.backward(2 * (y - target) / (t2[:, None, None, None] * B))
+
+    def backward(self, gy):
+        """Parameter gradient for the output gradient gy = dL/dy (B, H, W, c_out) of the last
+        forward pass (stored in bf16 like the device's when bf16_storage)."""
+        q, L, A, theta = self.q, self.L, self.A, self.theta_used
+        y = A["y"]
+        gy = q(gy)
         grad = np.zeros(self.n_params)
 
         def put(name, dW, db):
@@ -214,7 +220,7 @@ class UNetOracle:
             for name in self.layout:
                 b_off, co = self.layout[name][1], self.layout[name][3]
                 grad[b_off:b_off + co] = 0.0
-        return loss, grad
+        return grad
 
 
 def init_params(levels, channels, c_in, c_out, relu, seed=0, gain=None):
@@ -230,3 +236,98 @@ def init_params(levels, channels, c_in, c_out, relu, seed=0, gain=None):
         W[:co, :9 * ci] = rng.standard_normal((co, 9 * ci)) * std
         theta[w_off:w_off + cop * wk] = W.ravel()
     return theta
+
+
+# ---- the preconditioner map and the training losses of apx_unet_train_step ---------------------
+
+LOSS_KINDS = {"residual": 0, "energy": 1, "anorm": 2}
+
+
+def train_step(orc, theta, A, red2full, r_list, e_list=None, g_list=None, kind="anorm", w_e=0.0,
+               jacobi_skip=False, quant=True):
+    """Loss and parameter gradient of one training step of the learned preconditioner
+    (apx_unet_train_step), restated from the device's definitions:
+
+    - D = |diag A|, dsq = D^-1/2 per reduced DOF (1 where the diagonal is 0); the matrix sign
+      sA = sign(A_00) (definite matrices only; the Q1 tangents of the structured mesher are
+      negative definite);
+    - per image b: s_b = RMS(dsq r_b) rounded to FP32; the network input is dsq r_b / s_b at the
+      free DOFs of the node grid (node-major, component fastest, zero at Dirichlet DOFs), FP32
+      then bf16, plus the nodal field g_b as the last channel when the network has d + 1 inputs;
+    - z_b = dsq s_b y_b (s_b y_b formed in FP32) on the free DOFs, plus dsq^2 r_b with the Jacobi
+      skip; w = A z_b, res = r_b - w;
+    - kind 'residual': L_b = ||res||^2 / ||r_b||^2;  'energy': L_b = sA (z.w / 2 - z.r_b) / (s_b^2 n);
+      'anorm': L_b = (z_b - e_b).sA A (z_b - e_b) / e_b.sA A e_b written as
+      -sA (z_b - e_b).res / (sA e_b.r_b) (exact when A e_b = r_b);  each + w_e ||z - e||^2 / ||e||^2
+      when e is given;  loss = mean_b L_b;
+    - dL/dy_b = s_b dsq dL_b/dz_b / B, stored in bf16, then the network's backward pass.
+
+    quant=False keeps every value in FP64 (no FP32 / bf16 rounding), so the loss is a smooth
+    function of theta and its gradient can be checked by finite differences.
+    Returns (loss, grad, z_list)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    red2full = np.asarray(red2full, dtype=np.int64)
+    H, W = orc.H[0], orc.W[0]
+    d = orc.c_out
+    B = len(r_list)
+    diag = A.diagonal()
+    ad = np.abs(diag)
+    dsq = np.where(ad > 0, 1.0 / np.sqrt(np.where(ad > 0, ad, 1.0)), 1.0)
+    sA = -1.0 if diag[0] < 0 else 1.0
+    f32 = (lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)) if quant else (lambda a: np.asarray(a, dtype=np.float64))
+    x0 = np.zeros((B, H, W, orc.c_in))
+    scales = []
+    for b, r in enumerate(r_list):
+        v = r * dsq
+        rms = np.sqrt(np.sum(v * v) / max(n, 1))
+        sb = float(f32(rms)) if rms > 0 else 1.0
+        scales.append(sb)
+        inv = float(f32(1.0 / sb)) if quant else 1.0 / sb
+        xin = f32(f32(v) * inv)
+        flat = np.zeros(H * W * d)
+        flat[red2full] = xin
+        x0[b, :, :, :d] = flat.reshape(H, W, d)
+        if orc.c_in > d:
+            x0[b, :, :, d] = f32(np.asarray(g_list[b])).reshape(H, W)
+    y = orc.forward(theta, x0)
+    gy = np.zeros_like(y)
+    loss = 0.0
+    zs = []
+    for b, r in enumerate(r_list):
+        sb = scales[b]
+        yb = y[b].reshape(-1)[red2full]
+        z = f32(sb * f32(yb)) * dsq
+        if jacobi_skip:
+            z = z + r * dsq * dsq
+        zs.append(z)
+        w = A @ z
+        res = r - w
+        e = None if e_list is None else e_list[b]
+        ee = max(float(e @ e), 1e-300) if e is not None else 1.0
+        if kind == "residual":
+            rr = max(float(r @ r), 1e-300)
+            Lb = float(res @ res) / rr
+            gz = -2.0 * (A.T @ res) / rr
+        elif kind == "energy":
+            norm = sb * sb * n
+            Lb = sA * (0.5 * float(z @ w) - float(z @ r)) / norm
+            gz = -sA * res / norm
+        elif kind == "anorm":
+            if e is None:
+                raise ValueError("the A-norm loss needs the exact errors e")
+            E = max(sA * float(e @ r), 1e-300)
+            Lb = -sA * (float(z @ res) - float(e @ res)) / E
+            gz = -2.0 * sA * res / E
+        else:
+            raise ValueError(kind)
+        if e is not None and (w_e > 0 or kind == "anorm"):
+            Lb += w_e * float((z - e) @ (z - e)) / ee
+            gz = gz + 2.0 * w_e * (z - e) / ee
+        loss += Lb
+        g = np.zeros(H * W * d)
+        g[red2full] = f32(sb * gz * dsq / B)
+        gy[b] = g.reshape(H, W, d)
+    grad = orc.backward(gy)
+    return loss / B, grad, zs

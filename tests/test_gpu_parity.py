@@ -201,6 +201,50 @@ def test_tapered_panel_traction_matches_oracle(torch_gpu, precision):
     assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
 
 
+def test_linear_solver_type_matches_oracle_solve_linear(torch_gpu):
+    """'solver type': 'linear' (solver.py:544-659) from a nonzero starting point with nonzero
+    Dirichlet values: one linear solve at the imposed DOFs; the result holds the Dirichlet values
+    at Dirichlet DOFs and the increment at free DOFs (P5), and infos is None (solver.py:41-137).
+    Against the oracle's solve_linear with SuperLU, to 1e-8."""
+    from autopdex_amd import solver
+    from oracle import assembly as OA
+    from oracle import newton as ON
+    dofs, settings, static, o = P.poisson_source(24, dict_dofs=False)
+    mask = o["mask"]
+    rng = np.random.default_rng(12)
+    values = np.where(mask, rng.standard_normal(mask.size), 0.0)
+    u0 = 0.1 * rng.standard_normal(mask.size)
+    settings = dict(settings, **{"dirichlet conditions": values})
+    st = dict(static, **{"solver type": "linear", "linear tolerance": 1e-13})
+    sol, infos = solver.solver(u0, settings, st)
+    assert infos is None
+    ref = ON.solve_linear(u0, lambda u: OA.residual(o["sets"], o["coords"], u, 1),
+                          lambda u: OA.reduced_tangent(o["sets"], o["coords"], u, 1, ~mask), mask, values,
+                          lambda A, b: ON.linear_solve(A, b, "lapack")[0])
+    assert np.array_equal(sol.ravel()[mask], values[mask])
+    assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_damped_newton_solver_type_matches_oracle(torch_gpu):
+    """'solver type': 'damped newton' (solver.py:770-835, damped_newton :837-948) with the
+    reference's default damping 0.8 on a Neo-Hooke cube: the same Newton count and solution as the
+    oracle's damped Newton."""
+    from autopdex_amd import solver
+    dofs, settings, static, o = P.neo_hooke_cube(4, disp=0.05)
+    st = dict(static, **{"solver type": "damped newton", "linear tolerance": 1e-12})
+    sol, infos = solver.solver(dofs, settings, st, newton_tol=1e-9)
+    from oracle import assembly as OA
+    from oracle import newton as ON
+    free = ~o["mask"]
+    ref, rinfo = ON.newton(np.zeros(o["mask"].size), lambda u: OA.residual(o["sets"], o["coords"], u, 3),
+                           lambda u: OA.reduced_tangent(o["sets"], o["coords"], u, 3, free), o["mask"], o["values"],
+                           lambda A, b: ON.linear_solve(A, b, "lapack")[0], tol=1e-9, damping=0.8)
+    print(f"[case] damped newton: device {tuple(infos)} oracle {rinfo}", flush=True)
+    assert infos[2] is False and rinfo[2] is False
+    assert infos[0] == rinfo[0] and infos[0] > 3  # damping 0.8 converges linearly: more steps than Newton
+    assert np.linalg.norm(sol.ravel() - ref) / np.linalg.norm(ref) < 1e-8
+
+
 def test_adaptive_load_stepping_matches_oracle(torch_gpu):
     """solver.adaptive_load_stepping on the device against the oracle's restatement: a Neo-Hooke
     cube pulled to u_x = 2 with a first increment of 1 (which diverges) -- same multiplier

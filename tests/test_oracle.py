@@ -154,6 +154,10 @@ def _torch_potential(torch, xe, ue, kind, params):
             x2 = x - torch.tensor([1.0, 0.5], dtype=torch.float64)
             b = 20 * (torch.sin(10 * (x * x).sum(-1)) - torch.cos(10 * (x2 * x2).sum(-1)))
             psi = 0.5 * (grad * grad).sum(-1) - b * (ue @ N)
+        elif kind == "vark":
+            # 1/2 k |grad u|^2 - f u with k = exp(N.g), f = N.f from nodal g, f
+            ge, fe = (torch.tensor(a) for a in params)
+            psi = 0.5 * torch.exp(ge @ N) * (grad * grad).sum(-1) - (fe @ N) * (ue @ N)
         else:
             lam, mu = params
             F = torch.eye(dim, dtype=torch.float64) + grad
@@ -165,12 +169,13 @@ def _torch_potential(torch, xe, ue, kind, params):
 
 
 @pytest.mark.parametrize("builder,args,kind", [(P.poisson_source, (4,), "poisson"),
+                                               (P.poisson_vark, (4, 2), "vark"),
                                                (P.neo_hooke_cube, (2,), "neo"),
                                                (P.tapered_panel, (4,), "neo")])
 def test_closed_forms_match_autograd_of_reference_potential(builder, args, kind):
     torch = pytest.importorskip("torch")
     model, xe, ue, ne, d = _elements(builder, *args)
-    params = (getattr(model, "lam", 0.0), getattr(model, "mu", 0.0))
+    params = ne if kind == "vark" else (getattr(model, "lam", 0.0), getattr(model, "mu", 0.0))
     X = torch.tensor(xe)
     U = torch.tensor(ue, requires_grad=True)
     pot = _torch_potential(torch, X, U, kind, params)
@@ -246,6 +251,43 @@ def test_reduced_csr_equals_bruteforce_dense(builder, args):
     K = sum(OA.dense_bruteforce(m, o["coords"], c, u, o["d"], nd) for m, c, nd in o["sets"])
     free = ~o["mask"]
     assert np.allclose(A, K[np.ix_(free, free)], atol=1e-14 * np.abs(K).max())
+
+
+@pytest.mark.parametrize("builder,args", [(P.poisson_source, (5,)), (P.poisson_vark, (5, 4)),
+                                          (P.tapered_panel, (3,)), (P.neo_hooke_cube, (2,))])
+def test_diagonal_equals_bruteforce_dense_diagonal(builder, args):
+    """assembler.assemble_tangent_diagonal (assembler.py:639-680): the oracle's diagonal is the
+    diagonal of the brute-force dense tangent (full length, Dirichlet rows included), summed over
+    the sets."""
+    dofs, settings, static, o = builder(*args)
+    u = np.random.default_rng(3).standard_normal(o["mask"].size) * 0.05
+    g = OA.diagonal(o["sets"], o["coords"], u, o["d"])
+    K = sum(OA.dense_bruteforce(m, o["coords"], c, u, o["d"], nd) for m, c, nd in o["sets"])
+    assert g.shape == (o["mask"].size,)
+    assert np.allclose(g, np.diag(K), rtol=1e-14, atol=1e-15 * np.abs(K).max())
+
+
+def test_fcg_with_a_fixed_spd_preconditioner_is_pcg():
+    """Flexible CG with a fixed SPD preconditioner makes the iterates of preconditioned CG (the
+    Polak-Ribiere beta reduces to Fletcher-Reeves because z_k . r_(k-1) = 0): same iteration
+    count and solution as the oracle PCG (itself pinned to SuperLU and scipy cg).  With a
+    preconditioner that changes between iterations (a few inner Jacobi-PCG steps, nonlinear in
+    r) it still reaches the SuperLU solution."""
+    import scipy.sparse.linalg as spla
+    dofs, settings, static, o = P.poisson_source(24)
+    A = OA.reduced_tangent(o["sets"], o["coords"], np.zeros(o["mask"].size), 1, ~o["mask"])
+    A = -A  # the Q1 tangent of the structured mesher is negative definite (P2)
+    b = np.random.default_rng(4).standard_normal(A.shape[0])
+    d = A.diagonal()
+    x_p, k_p = ON.pcg(A, b, lambda v: v / d, tol=1e-10)
+    x_f, k_f = ON.fcg(A, b, lambda v: v / d, tol=1e-10, maxiter=5000)
+    assert abs(k_f - k_p) <= 1
+    assert np.linalg.norm(x_f - x_p) / np.linalg.norm(x_p) < 1e-8
+    x_lu = spla.spsolve(A.tocsc(), b)
+    inner = lambda v: ON.pcg(A, v, lambda w: w / d, tol=0.0, maxiter=3)[0]
+    x_v, k_v = ON.fcg(A, b, inner, tol=1e-10, maxiter=5000)
+    assert np.linalg.norm(x_v - x_lu) / np.linalg.norm(x_lu) < 1e-8
+    assert k_v < k_p
 
 
 def test_pcg_matches_superlu_and_scipy_cg():
