@@ -62,6 +62,7 @@ _SIGS = {
     "apx_assemble_tangent_coo": (_int, [_vp, _vp, _vp, _int, _vp]),
     "apx_spmv": (_int, [_vp, _vp, _vp, _vp, _int, _vp]),
     "apx_pcg": (_int, [_vp, _vp, _vp, _vp, _int, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
+    "apx_pcg_noalloc": (_int, [_vp, _vp, _vp, _vp, _int, _int, _dbl, _dbl, _int, C.POINTER(KrylovStats), _vp]),
     "apx_newton": (_int, [_vp, _vp, _vp, C.POINTER(NewtonOptions), C.POINTER(NewtonStats), _vp]),
     "apx_unet_create": (_int, [C.POINTER(_vp), _int, _vp, _int, _int, _int, _int, _int, _int]),
     "apx_unet_destroy": (_int, [_vp]),

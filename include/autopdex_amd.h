@@ -153,6 +153,20 @@ typedef struct apx_krylov_stats {
 int apx_pcg(apx_plan* plan, const void* vals_dev, const void* b_dev, void* x_dev, int precision,
             int precond, double tol, double atol, int maxiter, apx_krylov_stats* stats, void* stream);
 
+/* Allocation-free apx_pcg.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels, iterates and
+ * results as apx_pcg, but everything apx_pcg allocates (the SELL copy of the
+ * values, the inverse diagonal, six Krylov vectors, reduction partials, device
+ * scalars) lives in one 32 MiB device array per GPU, held until the process ends;
+ * the stop flag is polled without events.  A solve needs
+ * 256 + 262144 + round256(sell_entries * T) + round256(n_free * T) +
+ * round256(6 n_free * T) bytes (T = 8 FP64, 4 FP32); a larger one returns
+ * APX_ERR_ARG.  All calls on one GPU share its array: calls that overlap on one
+ * GPU race.                                                                  */
+int apx_pcg_noalloc(apx_plan* plan, const void* vals_dev, const void* b_dev, void* x_dev, int precision,
+                    int precond, double tol, double atol, int maxiter, apx_krylov_stats* stats, void* stream);
+
 /* ---- Newton ---------------------------------------------------------------- */
 
 typedef struct apx_newton_options {
