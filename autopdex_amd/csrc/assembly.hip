@@ -104,6 +104,7 @@ void fill(T* p, int64_t n, T v, hipStream_t s) {
 }
 template void fill<double>(double*, int64_t, double, hipStream_t);
 template void fill<float>(float*, int64_t, float, hipStream_t);
+template void fill<int>(int*, int64_t, int, hipStream_t);
 
 __global__ void k_round_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -220,11 +221,11 @@ int64_t residual_bytes(const Plan& P) {
 
 int64_t tangent_bytes(const Plan& P, int precision) {
   const int vb = precision == APX_FP64 ? 8 : 4;
-  int64_t b = P.n_nodes * P.dim * 8 + P.n_dofs * 8;
-  if (tangent_by_tiles(P))
-    b += P.sell_total * vb + P.n_free * 4 + (P.n_chunks + 1) * 4 + P.n_dofs;  // values incl. padding, maps, mask
+  int64_t b = P.n_nodes * P.dim * 8;
+  if (tangent_by_tiles(P))  // values incl. padding, maps, mask; the tile models' tangent does not read u
+    b += P.sell_total * vb + P.n_free * 4 + (P.n_chunks + 1) * 4 + P.n_dofs;
   else
-    b += P.nnz * vb;
+    b += P.n_dofs * 8 + P.nnz * vb;
   for (auto& s : P.sets) {
     if (!tangent_by_tiles(P)) b += s.n_elems * s.npe * 4;
     if (s.model == APX_MODEL_POISSON_VARK) b += P.n_nodes * 16;

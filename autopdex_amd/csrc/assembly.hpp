@@ -42,11 +42,8 @@ void spmv(Plan& P, const void* vals, const void* x, void* y, int precision, hipS
 void sell_to_csr(Plan& P, const void* sell, void* csr, int precision, hipStream_t st);
 void csr_to_sell(Plan& P, const void* csr, void* sell, int precision, hipStream_t st);
 void jacobi_inverse(Plan& P, const void* vals, void* inv_diag, int precision, hipStream_t st);
-// FP64 inverse diagonal of an FP32 SELL matrix (plus the diagonal correction rowdef when given)
-void jacobi_inverse_f32_to_f64(Plan& P, const float* vals, const float* rowdef, double* inv_diag, hipStream_t st);
-// rowdef_i = sum over row i's SELL entries of (v64 - v32) in FP64, stored in FP32: the row-sum
-// defect that rounding the FP64 matrix v64 to v32 introduced
-void row_defect(Plan& P, const double* v64, const float* v32, float* rowdef, hipStream_t st);
+// FP64 inverse diagonal of an FP32 SELL matrix
+void jacobi_inverse_f32_to_f64(Plan& P, const float* vals, double* inv_diag, hipStream_t st);
 
 // One preconditioned-CG solve on the plan's SELL matrix with values of type M
 // and Krylov vectors and scalars of type T (M = float, T = double: an FP32
@@ -63,9 +60,6 @@ struct PcgRun {
   int64_t n = 0;
   T *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *xw = nullptr, *x = nullptr;
   KState* S = nullptr;
-  // FP32 matrix with FP64 vectors only: the SpMV adds rowdef_i x_i, so the FP32 matrix keeps
-  // the FP64 row sums (null: no correction)
-  const float* rowdef = nullptr;
   double* part = nullptr;  // reduction partials
   // external = true: workspace, partials and scalars come from the caller, and the run
   // creates no events, records no timing and polls the stop flag synchronously (allocates

@@ -190,46 +190,25 @@ void csr_to_sell(Plan& P, const void* csr, void* sell, int precision, hipStream_
 
 // ---- Jacobi ------------------------------------------------------------------------
 template <class M, class V = M>
-__global__ void k_inv_diag(int64_t n, const int32_t* diag_slot, const M* vals, const float* rowdef, V* inv) {
+__global__ void k_inv_diag(int64_t n, const int32_t* diag_slot, const M* vals, V* inv) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    inv[i] = V(1) / ((V)vals[diag_slot[i]] + (rowdef ? (V)rowdef[i] : V(0)));
-}
-
-// Row-sum defect of a rounded SELL matrix, one lane per row (padding entries are zero in both).
-__global__ __launch_bounds__(512) void k_row_defect(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
-                                                    const double* __restrict__ v64, const float* __restrict__ v32,
-                                                    float* __restrict__ rowdef) {
-  const int lane = threadIdx.x & 63;
-  for (int64_t c = blockIdx.x * 8 + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * 8) {
-    const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
-    double s = 0.0;
-    for (int k = 0; k < w; ++k) s += v64[off + k * 64 + lane] - (double)v32[off + k * 64 + lane];
-    const int64_t row = c * 64 + lane;
-    if (row < n) rowdef[row] = (float)s;
-  }
-}
-
-void row_defect(Plan& P, const double* v64, const float* v32, float* rowdef, hipStream_t st) {
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((P.n_chunks + 7) / 8, 65536));
-  hipLaunchKernelGGL(k_row_defect, dim3(g), dim3(512), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p, v64, v32, rowdef);
-  APX_LAUNCH_CHECK();
+    inv[i] = V(1) / (V)vals[diag_slot[i]];
 }
 
 void jacobi_inverse(Plan& P, const void* vals, void* inv, int precision, hipStream_t st) {
   const int g = std::max(1, (int)std::min<int64_t>((P.n_free + 255) / 256, 65536));
   if (precision == APX_FP64)
     hipLaunchKernelGGL(k_inv_diag<double>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const double*)vals,
-                       nullptr, (double*)inv);
+                       (double*)inv);
   else
     hipLaunchKernelGGL(k_inv_diag<float>, dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, (const float*)vals,
-                       nullptr, (float*)inv);
+                       (float*)inv);
   APX_LAUNCH_CHECK();
 }
 
-void jacobi_inverse_f32_to_f64(Plan& P, const float* vals, const float* rowdef, double* inv, hipStream_t st) {
+void jacobi_inverse_f32_to_f64(Plan& P, const float* vals, double* inv, hipStream_t st) {
   const int g = std::max(1, (int)std::min<int64_t>((P.n_free + 255) / 256, 65536));
-  hipLaunchKernelGGL((k_inv_diag<float, double>), dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, vals, rowdef,
-                     inv);
+  hipLaunchKernelGGL((k_inv_diag<float, double>), dim3(g), dim3(256), 0, st, P.n_free, P.diag_slot.p, vals, inv);
   APX_LAUNCH_CHECK();
 }
 
@@ -294,8 +273,7 @@ template <class M, class T>
 __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
                                                       const int32_t* __restrict__ cols, const M* __restrict__ vals,
                                                       const T* __restrict__ z, T* __restrict__ pa, T* __restrict__ pb,
-                                                      T* __restrict__ q, const float* __restrict__ rowdef, KState* S,
-                                                      double* partials) {
+                                                      T* __restrict__ q, KState* S, double* partials) {
   if (halted(S)) return;
   constexpr int wpb = kSpmvBS / 64;
   const T beta = (T)S->beta;
@@ -311,10 +289,9 @@ __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunk
     const int64_t row = c * 64 + lane;
     if (row < n) {
       const T pi = z[row] + beta * po[row];
-      const T qi = rowdef ? s + (T)rowdef[row] * pi : s;
       pn[row] = pi;
-      q[row] = qi;
-      v[0] += (double)pi * (double)qi;
+      q[row] = s;
+      v[0] += (double)pi * (double)s;
     }
   }
   block_partials<kSpmvBS, 1>(v, partials);
@@ -447,8 +424,6 @@ PcgRun<M, T>::~PcgRun() {
 template <class M, class T>
 void PcgRun<M, T>::init(const T* b, T* x_, double tol, double atol, int maxiter_, double delta) {
   x = x_ ? x_ : xw;
-  spmv_bytes = P.sell_total * (int64_t)(sizeof(M) + 4) + (P.n_chunks + 1) * 4 + n * 4 * (int64_t)sizeof(T) +
-               (sizeof(M) < sizeof(T) && rowdef ? n * 4 : 0);
   maxiter = maxiter_ <= 0 ? (int)std::min<int64_t>(10 * n, INT32_MAX / 2) : maxiter_;
   const int g = grid_for(n);
   hipLaunchKernelGGL(k_pcg_init<T>, dim3(g), dim3(kBS), 0, st, n, b, inv, x, r, z, pa, S, part, tol, atol,
@@ -463,7 +438,7 @@ void PcgRun<M, T>::launch_iteration() {
   const bool timed = P.timing && !external;
   EventPair e1 = timed ? timing_begin(P, st) : EventPair{};
   hipLaunchKernelGGL((k_pcg_spmv<M, T>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals,
-                     z, pa, pb, q, sizeof(M) < sizeof(T) ? rowdef : nullptr, S, part);
+                     z, pa, pb, q, S, part);
   if (timed) timing_end(P, e1, st);
   hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gs);
   EventPair e2 = timed ? timing_begin(P, st) : EventPair{};

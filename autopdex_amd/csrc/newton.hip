@@ -63,13 +63,6 @@ struct NewtonCtx {
   // rounded one (Jacobi-PCG: 20692 and 13456); one FP64 assembly plus the rounding cost far less
   // than one network application.
   bool fp32_arith = false;  // FP32 tangent: element arithmetic in FP32 instead of rounding the FP64 one
-  // Row-sum defect of the rounded FP32 matrix (FP32 matrix with FP64 Krylov vectors, rounded
-  // from the FP64 tangent), or null.  Rounding every entry of the Poisson stencil to FP32 leaves
-  // each row sum off by ~1e-7 |A|, a diagonal perturbation of ~2 % of the smallest eigenvalue at
-  // 2048^2: CG on the rounded matrix drifts from the FP64 residual on smooth modes.  Adding the
-  // defect as a diagonal term (4 bytes per row in the SpMV) restores the FP64 row sums, so the
-  // remaining perturbation acts on rough modes only.
-  const float* rowdef = nullptr;
   // Tangent values of type M into vals and the preconditioner's inverse diagonal of type V
   // (an FP32 matrix may run with FP64 Krylov vectors: its diagonal is widened exactly).
   template <class M, class V>
@@ -77,16 +70,10 @@ struct NewtonCtx {
     const int prec = sizeof(M) == 8 ? APX_FP64 : APX_FP32;
     auto t0 = std::chrono::steady_clock::now();
     EventPair e = timing_begin(P, st);
-    rowdef = nullptr;
     if (prec == APX_FP32 && !fp32_arith) {
       if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
       assemble_tangent(P, u, P.vals64.p, APX_FP64, st);
       round_to_f32(P.vals64.p, (float*)vals, P.sell_total, st);
-      if constexpr (sizeof(M) < sizeof(V)) {
-        if (P.rowdef.n < (size_t)P.n_free) P.rowdef.alloc(P.n_free);
-        row_defect(P, P.vals64.p, (const float*)vals, P.rowdef.p, st);
-        rowdef = P.rowdef.p;
-      }
     } else {
       assemble_tangent(P, u, vals, prec, st);
     }
@@ -96,7 +83,7 @@ struct NewtonCtx {
       if constexpr (sizeof(M) == sizeof(V))
         jacobi_inverse(P, vals, inv, prec, st);
       else
-        jacobi_inverse_f32_to_f64(P, vals, rowdef, inv, st);
+        jacobi_inverse_f32_to_f64(P, vals, inv, st);
     } else {
       fill<V>(inv, P.n_free, V(1), st);
     }
@@ -183,7 +170,6 @@ static void newton_standard(NewtonCtx& C, const double* dvals, const apx_newton_
   while (go) {
     if (!have_tangent || !o.reuse_tangent) {
       C.tangent(vals, inv, learned ? APX_PRECOND_JACOBI : o.precond);
-      pcg.rowdef = C.rowdef;
       have_tangent = true;
     }
     double eta = o.lin_tol;
@@ -249,7 +235,6 @@ static void newton_reliable(NewtonCtx& C, const double* dvals, const apx_newton_
   gather_rhs(P, C.r, b, vprec, C.st);
   auto t0 = std::chrono::steady_clock::now();
   PcgRun<float, V> pcg(P, vals, inv, C.st);
-  pcg.rowdef = C.rowdef;
   // the FP32 recursion stops on its own at half the Newton threshold; the
   // FP64 residual decides at every anchor
   pcg.init(b, nullptr, 0.0, 0.5 * thr, o.lin_maxiter, o.reliable_delta);

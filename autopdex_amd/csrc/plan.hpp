@@ -33,6 +33,7 @@ struct ElemSet {
   int64_t gnx = 0, gny = 0;
   DevBuf<double> load;     // u-independent residual part (precomputed-source models), n_dofs
   bool load_valid = false;
+  int all_parallelograms = -1;  // structured grid of parallelograms only: -1 not yet checked, 0 no, 1 yes
 };
 
 // Timing of a kernel family: HIP events on the caller's stream.
@@ -78,9 +79,6 @@ struct Plan {
   // workspace (lazily sized)
   DevBuf<double> vals64, w64;  // nnz, vectors
   DevBuf<float> vals32, w32;
-  // row-sum defect of the FP32 matrix rounded from the FP64 one: rowdef_i = sum_j (A64_ij -
-  // A32_ij), n_free (the diagonal correction of the FP32-matrix SpMV with FP64 vectors)
-  DevBuf<float> rowdef;
   DevBuf<double> r_full, diag_full;  // n_dofs
   DevBuf<double> partials;     // reduction partials
   DevBuf<double> tw, tpart;    // network training workspace (4 n_free) and its reduction partials
@@ -97,7 +95,7 @@ struct Plan {
   // (name = its template, e.g. "k_conv3x3<64, 64, false, 4>") for the names in timing_filter;
   // 2 every network kernel
   int timing = 0;
-  std::string timing_filter;  // comma-separated kernel names timed per launch at level 1
+  std::string timing_filter;  // ';'-separated kernel names timed per launch at level 1
   std::map<std::string, KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> evpool;  // recycled timing events

@@ -37,6 +37,106 @@ constexpr int kTBS = kNE;
 
 enum { TILE_RES = 0, TILE_TAN = 1, TILE_DIAG = 2 };
 
+// ---- parallelogram fast path for the Poisson models ---------------------------------
+// A Q1 element whose nodes form a parallelogram (x0 - x1 + x2 - x3 = 0) has a constant
+// Jacobian, so every Gauss point shares det J and B = J^-1 J^-T, and the 2 x 2-point
+// stiffness is K_ab = det J sum_q k_q sum_jl dN_q[a][j] B_jl dN_q[b][l] with the reference
+// gradients dN_q of Shape<4, 2> (constants).  The mesher's grids of rectangles and
+// parallelograms take this path: one 2 x 2 inverse per element instead of four, and no
+// per-point gradient.  Elements that deviate by more than 1e-13 of their size use the
+// general per-point evaluation; within that bound J differs between the points by at most
+// 1e-13 relative, below the FP64 agreement the element tests ask for.
+__host__ __device__ constexpr double q1_gx(int i) { return i ? kGaussB : kGaussA; }
+// dN_a/dxi_i at Gauss point q, the arithmetic of Shape<4, 2>::eval
+__host__ __device__ constexpr double q1_dN(int q, int a, int i) {
+  const double x0 = 0.5 * q1_gx(q & 1), x2 = 0.5 * q1_gx(q >> 1);
+  const double x1 = 0.5 - x0, x3 = 0.5 - x2, x4 = x0 + 0.5, x5 = x2 + 0.5;
+  switch (a * 2 + i) {
+    case 0: return -0.5 * x3;
+    case 1: return -0.5 * x1;
+    case 2: return 0.5 * x3;
+    case 3: return -0.5 * x4;
+    case 4: return 0.5 * x5;
+    case 5: return 0.5 * x4;
+    case 6: return -0.5 * x5;
+    default: return 0.5 * x1;
+  }
+}
+__host__ __device__ constexpr double q1_N(int q, int a) {
+  const double x0 = 0.5 * q1_gx(q & 1), x2 = 0.5 * q1_gx(q >> 1);
+  const double x1 = 0.5 - x0, x3 = 0.5 - x2, x4 = x0 + 0.5, x5 = x2 + 0.5;
+  return a == 0 ? x1 * x3 : a == 1 ? x3 * x4 : a == 2 ? x4 * x5 : x1 * x5;
+}
+// sum over the Gauss points (weights 1) of dN_q[a][j] dN_q[b][l] + (j != l: the transposed term)
+__host__ __device__ constexpr double q1_S(int a, int b, int j, int l) {
+  double s = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    s += q1_dN(q, a, j) * q1_dN(q, b, l);
+    if (j != l) s += q1_dN(q, a, l) * q1_dN(q, b, j);
+  }
+  return s;
+}
+
+// det J and B = J^-1 J^-T (B01 = B10) of a parallelogram element; false otherwise
+__device__ inline bool q1_parallelogram(const double (*x)[2], double& det, double& B00, double& B01, double& B11) {
+  const double h0 = x[0][0] - x[1][0] + x[2][0] - x[3][0], h1 = x[0][1] - x[1][1] + x[2][1] - x[3][1];
+  // J_ij = sum_a x_ai dN_aj at the centre, dN(0) = 0.25 (-1,-1), (1,-1), (1,1), (-1,1)
+  const double J00 = 0.25 * (-x[0][0] + x[1][0] + x[2][0] - x[3][0]);
+  const double J01 = 0.25 * (-x[0][0] - x[1][0] + x[2][0] + x[3][0]);
+  const double J10 = 0.25 * (-x[0][1] + x[1][1] + x[2][1] - x[3][1]);
+  const double J11 = 0.25 * (-x[0][1] - x[1][1] + x[2][1] + x[3][1]);
+  const double size = fabs(J00) + fabs(J01) + fabs(J10) + fabs(J11);
+  if (!(fabs(h0) + fabs(h1) <= 1e-13 * size)) return false;
+  det = J00 * J11 - J01 * J10;
+  const double id = 1.0 / det;
+  const double i00 = J11 * id, i01 = -J01 * id, i10 = -J10 * id, i11 = J00 * id;
+  B00 = i00 * i00 + i01 * i01;
+  B01 = i00 * i10 + i01 * i11;
+  B11 = i10 * i10 + i11 * i11;
+  return true;
+}
+
+// Element stiffness of a parallelogram in T: constant coefficient (KQ = false) from the
+// summed tables, or k_q per Gauss point
+template <class T, bool KQ>
+__device__ inline void q1_affine_stiffness(double det, double B00, double B01, double B11, const double* kq,
+                                           T K[4][4]) {
+  const T d = (T)det, b00 = (T)B00, b01 = (T)B01, b11 = (T)B11;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b) {
+      T v;
+      if constexpr (!KQ) {
+        v = d * (b00 * (T)q1_S(a, b, 0, 0) + b01 * (T)q1_S(a, b, 0, 1) + b11 * (T)q1_S(a, b, 1, 1));
+      } else {
+        v = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const T c00 = (T)(q1_dN(q, a, 0) * q1_dN(q, b, 0)), c11 = (T)(q1_dN(q, a, 1) * q1_dN(q, b, 1));
+          const T c01 = (T)(q1_dN(q, a, 0) * q1_dN(q, b, 1) + q1_dN(q, a, 1) * q1_dN(q, b, 0));
+          v += (T)kq[q] * (b00 * c00 + b01 * c01 + b11 * c11);
+        }
+        v *= d;
+      }
+      K[a][b] = v;
+      K[b][a] = v;
+    }
+}
+
+template <class M>
+struct AffineModel {  // which models take the parallelogram path, and how
+  static constexpr bool ok = false;
+};
+template <>
+struct AffineModel<PoissonSource<4, 2>> {
+  static constexpr bool ok = true, nodal = false;
+};
+template <>
+struct AffineModel<PoissonVarK<4, 2>> {
+  static constexpr bool ok = true, nodal = true;
+};
+
 struct TileArgs {
   const double* coords;  // SoA, 2 x n_nodes
   int64_t n_nodes;
@@ -54,17 +154,48 @@ struct TileArgs {
   int64_t tiles, tiles_j;  // tiles in the grid, tiles per tile row
 };
 
-template <class M, class T, int MODE>
+// index of K_ab (a <= b, or swapped) in the upper triangle of a symmetric 4 x 4 matrix
+__host__ __device__ constexpr int sym_entry(int a, int b) {
+  return a <= b ? a * 4 - a * (a - 1) / 2 + (b - a) : sym_entry(b, a);
+}
+
+// an element matrix into the LDS planes: the upper triangle (tangent) or the diagonal
+template <int MODE, class T>
+__device__ inline void store_matrix(T* er, int el, const T (&K)[4][4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if constexpr (MODE == TILE_DIAG) {
+      er[a * kNE + el] = K[a][a];
+    } else {
+#pragma unroll
+      for (int b = a; b < 4; ++b) er[sym_entry(a, b) * kNE + el] = K[a][b];
+    }
+  }
+}
+
+// AFF: every element of the grid is a parallelogram (checked once per plan by
+// k_parallelogram_check with the same predicate), so only the parallelogram path is compiled;
+// the general per-point path holds registers that cost the kernel occupancy.
+template <class M, class T, int MODE, bool AFF>
 __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   constexpr int D = M::D, NPE = 4;
   static_assert(M::NPE == 4 && M::DIM == 2, "owner tiles are for Q1 quads");
   static_assert(MODE == TILE_RES || D == 1, "tangent / diagonal tiles: one DOF per node");
+  static_assert(!AFF || AffineModel<M>::ok, "the parallelogram-only kernel is for the Poisson models");
   __shared__ double sx[kNN][2];
-  __shared__ double su[kNN][D];
+  // the tile tangent models (Poisson, constant or nodal k) are linear in u: only the
+  // residual stages it
+  __shared__ double su[MODE == TILE_RES ? kNN : 1][D];
   __shared__ double sn[M::NODAL ? kNN : 1][2];
   __shared__ uint8_t sm[MODE == TILE_TAN ? kNN : 1];
-  constexpr int NR = (MODE == TILE_RES) ? kNE * NPE * D : kNE * NPE * NPE;
-  __shared__ T er[NR];  // residual rows er[el][a][c]; element matrices er[el][a][b]
+  // Element matrices of the tile models are symmetric, bit for bit (K_ab and K_ba are the same
+  // products summed in the same order), so the tangent keeps a <= b only and the diagonal
+  // keeps K_aa only.
+  constexpr int NE_ = MODE == TILE_RES ? NPE * D : MODE == TILE_TAN ? NPE * (NPE + 1) / 2 : NPE;
+  // one plane of kNE words per element entry: entry e of element el sits at er[e * kNE + el]
+  // (e = a * D + c for residual rows, sym_entry(a, b) for matrices, a for diagonals), so the
+  // lanes of a wave touch consecutive words in both phases and the LDS banks do not conflict
+  __shared__ T er[kNE * NE_];
 
   // XCD-aware tile order (1D grid rounded up to a multiple of 8): consecutive logical tiles,
   // row-major over the tile grid, share an L2
@@ -73,21 +204,70 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   const int64_t i0 = (lin / A.tiles_j) * kTI, j0 = (lin % A.tiles_j) * kTJ;
   const int64_t nx = A.nx, ny = A.ny;
   const int tid = threadIdx.x;
-  // stage the 9 x 65 nodes around the tile (rows of 65 consecutive nodes)
-  for (int idx = tid; idx < kNN; idx += kTBS) {
+  // owner-phase inputs first, so that their latency overlaps the staging loads and the
+  // element phase: thread tid < 441 owns node (i0 + tid / 63, j0 + tid % 63)
+  const int oli = tid / kTJ, olj = tid % kTJ;
+  const int64_t oI = i0 + oli, oJ = j0 + olj;
+  const bool owner = tid < kTI * kTJ && oI <= nx && oJ <= ny;
+  const int64_t onode = owner ? oI * (ny + 1) + oJ : 0;
+  double pl[D], pr[D];  // the node's load and previous r (residual), previous value (diagonal)
+#pragma unroll
+  for (int c = 0; c < D; ++c) pl[c] = pr[c] = 0.0;
+  int32_t orow = -1;  // reduced row of the node, -1 for a Dirichlet node
+  if (owner) {
+    if constexpr (MODE == TILE_TAN) {
+      orow = A.full2red[onode];
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        if (MODE == TILE_RES && A.load) pl[c] = A.load[onode * D + c];
+        if (!A.overwrite) pr[c] = A.r[onode * D + c];
+      }
+    }
+  }
+  // stage the 9 x 65 nodes around the tile (rows of 65 consecutive nodes): all loads of a
+  // thread are issued before its first LDS store
+  constexpr int kSR = (kNN + kTBS - 1) / kTBS;
+  double gx[kSR][2], gu[kSR][D], gn[kSR][2];
+  uint8_t gm[kSR];
+#pragma unroll
+  for (int q = 0; q < kSR; ++q) {
+    const int idx = tid + q * kTBS;
     const int li = idx / kNJ, lj = idx % kNJ;
     const int64_t I = i0 - 1 + li, J = j0 - 1 + lj;
-    const bool in = I >= 0 && I <= nx && J >= 0 && J <= ny;
+    const bool in = idx < kNN && I >= 0 && I <= nx && J >= 0 && J <= ny;
     const int64_t node = in ? I * (ny + 1) + J : 0;
-    sx[idx][0] = in ? A.coords[node] : 0.0;
-    sx[idx][1] = in ? A.coords[A.n_nodes + node] : 0.0;
+    gx[q][0] = in ? A.coords[node] : 0.0;
+    gx[q][1] = in ? A.coords[A.n_nodes + node] : 0.0;
 #pragma unroll
-    for (int c = 0; c < D; ++c) su[idx][c] = in ? A.u[node * D + c] : 0.0;
-    if constexpr (M::NODAL) {
-      sn[idx][0] = in ? A.mp.nodal[0][node] : 0.0;
-      sn[idx][1] = in ? A.mp.nodal[1][node] : 0.0;
+    for (int c = 0; c < D; ++c) gu[q][c] = (MODE == TILE_RES && in) ? A.u[node * D + c] : 0.0;
+    gn[q][0] = (M::NODAL && in) ? A.mp.nodal[0][node] : 0.0;
+    gn[q][1] = (M::NODAL && in) ? A.mp.nodal[1][node] : 0.0;
+    gm[q] = (MODE == TILE_TAN && in) ? (A.dmask[node] ? 1 : 0) : 2;  // 2: outside the grid
+  }
+#pragma unroll
+  for (int q = 0; q < kSR; ++q) {
+    const int idx = tid + q * kTBS;
+    if (idx >= kNN) continue;
+    sx[idx][0] = gx[q][0];
+    sx[idx][1] = gx[q][1];
+    if constexpr (MODE == TILE_RES) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) su[idx][c] = gu[q][c];
     }
-    if constexpr (MODE == TILE_TAN) sm[idx] = in ? (A.dmask[node] ? 1 : 0) : 2;  // 2: outside the grid
+    if constexpr (M::NODAL) {
+      sn[idx][0] = gn[q][0];
+      sn[idx][1] = gn[q][1];
+    }
+    if constexpr (MODE == TILE_TAN) sm[idx] = gm[q];
+  }
+  // the SELL chunk of the owned row, read while the elements are evaluated
+  int32_t obase = 0, owidth = 0;
+  if constexpr (MODE == TILE_TAN) {
+    if (orow >= 0) {
+      obase = A.sell_ptr[orow >> 6];
+      owidth = (A.sell_ptr[(orow >> 6) + 1] - obase) >> 6;
+    }
   }
   __syncthreads();
   // element phase: thread -> element (i0 - 1 + lei, j0 - 1 + lej); local node a of it sits at
@@ -105,7 +285,7 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
         xl[a][0] = sx[s][0];
         xl[a][1] = sx[s][1];
 #pragma unroll
-        for (int c = 0; c < D; ++c) ul[a][c] = su[s][c];
+        for (int c = 0; c < D; ++c) ul[a][c] = MODE == TILE_RES ? su[s][c] : 0.0;
         if constexpr (M::NODAL) {
           nl[a][0] = sn[s][0];
           nl[a][1] = sn[s][1];
@@ -113,7 +293,57 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
           nl[a][0] = nl[a][1] = 0.0;
         }
       }
-      if constexpr (MODE == TILE_RES) {
+      bool affine = false;
+      if constexpr (AffineModel<M>::ok) {
+        double det, B00, B01, B11;
+        // the residual of the source model with the source evaluated here needs the points
+        const bool par = q1_parallelogram(xl, det, B00, B01, B11);
+        if (AFF || ((MODE != TILE_RES || AffineModel<M>::nodal || A.want_source == 0) && par)) {
+          affine = true;
+          double kq[4] = {1.0, 1.0, 1.0, 1.0};
+          if constexpr (AffineModel<M>::nodal) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              double g = 0.0;
+#pragma unroll
+              for (int a = 0; a < NPE; ++a) g += q1_N(q, a) * nl[a][0];
+              kq[q] = exp(g);
+            }
+          }
+          if constexpr (MODE == TILE_RES) {
+            double K[4][4];
+            q1_affine_stiffness<double, AffineModel<M>::nodal>(det, B00, B01, B11, kq, K);
+#pragma unroll
+            for (int a = 0; a < NPE; ++a) {
+              double r = 0.0;
+#pragma unroll
+              for (int b = 0; b < NPE; ++b) r += K[a][b] * ul[b][0];
+              if constexpr (AffineModel<M>::nodal) {
+                // - det sum_q f_q N_q[a], f_q = N_q . f
+                double src = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  double f = 0.0;
+#pragma unroll
+                  for (int c = 0; c < NPE; ++c) f += q1_N(q, c) * nl[c][1];
+                  src += f * q1_N(q, a);
+                }
+                r -= det * src;
+              }
+              er[a * D * kNE + el] = r;
+            }
+          } else {
+            T K[4][4];
+            q1_affine_stiffness<T, AffineModel<M>::nodal>(det, B00, B01, B11, kq, K);
+            store_matrix<MODE>(er, el, K);
+          }
+        }
+      }
+      if constexpr (AFF) {
+        // every element takes the parallelogram path
+      } else if (affine) {
+        // element done by the parallelogram path
+      } else if constexpr (MODE == TILE_RES) {
         double out[NPE][D];
 #pragma unroll
         for (int a = 0; a < NPE; ++a)
@@ -128,7 +358,7 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
 #pragma unroll
         for (int a = 0; a < NPE; ++a)
 #pragma unroll
-          for (int c = 0; c < D; ++c) er[(el * NPE + a) * D + c] = out[a][c];
+          for (int c = 0; c < D; ++c) er[(a * D + c) * kNE + el] = out[a][c];
       } else {
         T K[NPE][NPE];
 #pragma unroll
@@ -141,20 +371,15 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
           M::template qp<T>(q, xl, ul, nl, A.mp, false, Q);
           M::template tan_gp<T>(Q, A.mp, K);
         }
-#pragma unroll
-        for (int a = 0; a < NPE; ++a)
-#pragma unroll
-          for (int b = 0; b < NPE; ++b) er[(el * NPE + a) * NPE + b] = K[a][b];
+        store_matrix<MODE>(er, el, K);
       }
     }
   }
   __syncthreads();
   // owner phase: thread -> node (i0 + li, j0 + lj)
-  if (tid >= kTI * kTJ) return;
-  const int li = tid / kTJ, lj = tid % kTJ;
-  const int64_t I = i0 + li, J = j0 + lj;
-  if (I > nx || J > ny) return;
-  const int64_t node = I * (ny + 1) + J;
+  if (!owner) return;
+  const int li = oli, lj = olj;
+  const int64_t I = oI, J = oJ;
   // the four elements around the node, in element order, with the node's local index:
   // (I-1, J-1): 2, (I-1, J): 3, (I, J-1): 1, (I, J): 0
   const bool ok[4] = {I >= 1 && J >= 1, I >= 1 && J < ny, I < nx && J >= 1, I < nx && J < ny};
@@ -166,10 +391,9 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        if (ok[k]) s += (double)er[(elx[k] * NPE + kLoc[k]) * D + c];
-      if (A.load) s += A.load[node * D + c];
-      double* ro = A.r + node * D + c;
-      *ro = A.overwrite ? s : *ro + s;
+        if (ok[k]) s += (double)er[(kLoc[k] * D + c) * kNE + elx[k]];
+      if (A.load) s += pl[c];
+      A.r[onode * D + c] = A.overwrite ? s : pr[c] + s;
     }
   } else {
     // v[dI + 1][dJ + 1]: coupling of the node with neighbour (I + dI, J + dJ)
@@ -185,17 +409,15 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
 #pragma unroll
       for (int b = 0; b < NPE; ++b) {
         const int dI = kEi[k] + kDi[b], dJ = kEjo[k] + kDj[b];
-        v[dI + 1][dJ + 1] += er[(elx[k] * NPE + kLoc[k]) * NPE + b];
+        if (MODE == TILE_DIAG && b != kLoc[k]) continue;
+        v[dI + 1][dJ + 1] += er[(MODE == TILE_DIAG ? kLoc[k] : sym_entry(kLoc[k], b)) * kNE + elx[k]];
       }
     }
     if constexpr (MODE == TILE_DIAG) {
-      double* ro = A.r + node;
-      *ro = A.overwrite ? (double)v[1][1] : *ro + (double)v[1][1];
+      A.r[onode] = A.overwrite ? (double)v[1][1] : pr[0] + (double)v[1][1];
     } else {
-      if (sm[(li + 1) * kNJ + lj + 1]) return;  // Dirichlet node: no reduced row
-      const int32_t row = A.full2red[node];
-      const int32_t base = A.sell_ptr[row >> 6], width = (A.sell_ptr[(row >> 6) + 1] - base) >> 6;
-      T* vals = (T*)A.vals + base + (row & 63);
+      if (orow < 0) return;  // Dirichlet node: no reduced row
+      T* vals = (T*)A.vals + obase + (orow & 63);
       int k = 0;
 #pragma unroll
       for (int dI = -1; dI <= 1; ++dI)
@@ -206,9 +428,46 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
             ++k;
           }
         }
-      for (; k < width; ++k) vals[(int64_t)k * 64] = 0;  // chunk padding
+      for (; k < owidth; ++k) vals[(int64_t)k * 64] = 0;  // chunk padding
     }
   }
+}
+
+// one thread per element of the structured grid: flag any element that is not a parallelogram
+__global__ void k_parallelogram_check(const double* coords, int64_t n_nodes, int64_t nx, int64_t ny, int* irregular) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nx * ny) return;
+  const int64_t b = (e / ny) * (ny + 1) + e % ny;
+  const int64_t nd[4] = {b, b + 1, b + ny + 2, b + ny + 1};  // local nodes 0..3 as in k_tile
+  double x[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    x[a][0] = coords[nd[a]];
+    x[a][1] = coords[n_nodes + nd[a]];
+  }
+  double det, B00, B01, B11;
+  if (!q1_parallelogram(x, det, B00, B01, B11)) *irregular = 1;  // every writer stores the same value
+}
+
+// Whether every element of the set is a parallelogram; decided on the device once per set (the
+// coordinates are fixed for the plan's lifetime).
+bool all_parallelograms(Plan& P, ElemSet& S, hipStream_t st) {
+  if (S.all_parallelograms < 0) {
+    DevBuf<int> flag;
+    flag.alloc(1);
+    fill<int>(flag.p, 1, 0, st);
+    const int64_t ne = S.gnx * S.gny;
+    if (ne > 0) {
+      hipLaunchKernelGGL(k_parallelogram_check, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, P.coords.p,
+                         P.n_nodes, S.gnx, S.gny, flag.p);
+      APX_LAUNCH_CHECK();
+    }
+    int h = 1;
+    APX_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    APX_HIP(hipStreamSynchronize(st));
+    S.all_parallelograms = h ? 0 : 1;
+  }
+  return S.all_parallelograms == 1;
 }
 
 template <class M, class T, int MODE>
@@ -237,7 +496,17 @@ void run_tile(Plan& P, ElemSet& S, const double* u, double* r, void* vals, const
   A.tiles = A.tiles_j * ((S.gnx + 1 + kTI - 1) / kTI);
   const int64_t g = round_xcd(A.tiles);
   APX_REQUIRE(g < ((int64_t)1 << 31), APX_ERR_UNSUPPORTED, "structured grid too large for the owner-tile launch");
-  hipLaunchKernelGGL((k_tile<M, T, MODE>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
+  bool aff = false;
+  if constexpr (AffineModel<M>::ok) {
+    // the source model's residual takes the parallelogram path only with its source part given
+    aff = (MODE != TILE_RES || AffineModel<M>::nodal || A.want_source == 0) && all_parallelograms(P, S, st);
+  }
+  if (aff) {
+    if constexpr (AffineModel<M>::ok)
+      hipLaunchKernelGGL((k_tile<M, T, MODE, true>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
+  } else {
+    hipLaunchKernelGGL((k_tile<M, T, MODE, false>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
+  }
   APX_LAUNCH_CHECK();
 }
 }  // namespace

@@ -931,10 +931,11 @@ void UNet::collect_timing(size_t count) {
 }
 
 namespace {
+// filter: kernel names separated by ';' (the names themselves contain commas)
 bool filter_has(const std::string& filter, const std::string& name) {
   size_t pos = 0;
   while (pos <= filter.size()) {
-    const size_t end = std::min(filter.find(',', pos), filter.size());
+    const size_t end = std::min(filter.find(';', pos), filter.size());
     if (filter.compare(pos, end - pos, name) == 0) return true;
     pos = end + 1;
   }

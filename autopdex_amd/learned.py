@@ -402,6 +402,98 @@ def fit_fields(problem, height, width, fields, set_index=0, levels=None, channel
     return inf, info
 
 
+def fit_field_family(problem, height, width, field, train_seeds, set_index=0, levels=None, channels=None,
+                     pair_tol=1e-6, steps=1500, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
+                     validation_seed=None, log=None):
+    """fit_fields for a large family of coefficient fields, streaming: field(seed) -> (g, f)
+    device tensors (FP64, one value per node) is called once per training seed to make two
+    (r, A^-1 r) pairs by plain Jacobi-PCG (to pair_tol) -- one white-noise and one smooth
+    right-hand side -- which are kept in FP32 with g; the field's tangent is re-assembled at
+    every training step that draws it (a few hundred microseconds at 1025 x 1025) instead of
+    being stored.  Each step trains on the two pairs of one field (batch 2, relative A-norm
+    loss with that field's matrix).  Returns (network with batch 1, attached with the validation
+    field's g when given, else the last field's; info dict)."""
+    torch = torch_cuda()
+    t0 = time.perf_counter()
+    L = levels or default_levels(height, width)
+    ch = tuple(channels or ((16, 32) + (64,) * 10)[:L])
+    free = torch.from_numpy(problem.free_dofs().astype(np.int64)).to(problem.device)
+    gen = torch.Generator(device=problem.device)
+    gen.manual_seed(int(seed))
+    lengths = (0.2, 0.05, 0.02, 0.005)
+    data = []
+    for k, s in enumerate(train_seeds):
+        g, f = field(s)
+        problem.bind_nodal_device(set_index, g, f)
+        vals = tangent(problem)
+        rhs = [torch.randn(problem.n_free, generator=gen, device=problem.device, dtype=torch.float64),
+               random_field(height, width, lengths[k % len(lengths)], seed=seed * 100003 + int(s))[free].contiguous()]
+        prs = []
+        for b in rhs:
+            e, _ = pcg(problem, vals, b, tol=pair_tol)
+            prs.append((b.float(), e.float()))
+        data.append((g, prs))
+        if log and (k + 1) % 250 == 0:
+            log(f"[fit_field_family] pairs of {k + 1}/{len(train_seeds)} fields, {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    d = problem.n_dofs // (height * width)
+    net = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=2, seed=seed + 1)
+    net.set_params(net.initial_params(seed + 1, up_scale=up_scale))
+    inf = UNetPreconditioner(height, width, L, ch, d + 1, d, relu=False, batch=1)
+    val = None
+    if validation_seed is not None:
+        gv, fv = field(validation_seed)
+        problem.bind_nodal_device(set_index, gv, fv)
+        val = (tangent(problem), gv, gv.new_empty(problem.n_free).normal_(generator=gen))
+    rng = np.random.default_rng(seed)
+    history, best, done = [], None, 0
+    t1 = time.perf_counter()
+    seg = validate_every if (validate_every and val is not None) else steps
+    r64 = [torch.empty(problem.n_free, dtype=torch.float64, device=problem.device) for _ in range(2)]
+    e64 = [torch.empty_like(r64[0]) for _ in range(2)]
+    while done < steps:
+        k = min(seg, steps - done)
+        for it in range(done, done + k):
+            g, prs = data[rng.integers(len(data))]
+            problem.bind_nodal_device(set_index, g, g)  # the tangent does not read f
+            vals = tangent(problem)
+            for j in range(2):
+                r64[j].copy_(prs[j][0])
+                e64[j].copy_(prs[j][1])
+            r_list = (C.c_void_p * 2)(*[t.data_ptr() for t in r64])
+            e_list = (C.c_void_p * 2)(*[t.data_ptr() for t in e64])
+            g_list = (C.c_void_p * 2)(g.data_ptr(), g.data_ptr())
+            lval = C.c_double()
+            _call("apx_unet_train_step", net.handle, problem.plan, _vp(vals), r_list, g_list, e_list,
+                  _LOSS_KINDS["anorm"], 0.0, C.byref(lval), current_stream())
+            warm = min(1.0, (it + 1) / 50)
+            net.adam(warm * lr * 0.1 ** (it / max(steps - 1, 1)))
+            history.append(lval.value)
+        done += k
+        theta = net.get_params()
+        if val is not None:
+            inf.set_params(theta)
+            inf.attach(problem, val[1])
+            _, ks = fcg(problem, val[0], val[2], tol=1e-6, maxiter=400)
+            score = ks.iterations if ks.converged else 10 ** 6
+            if log:
+                log(f"[fit_field_family] step {done}: loss {np.mean(history[-50:]):.3e}, held-out field FCG "
+                    f"{ks.iterations} it")
+        else:
+            score = 0
+        if best is None or score <= best[0]:
+            best = (score, theta, done)
+    torch.cuda.synchronize()
+    inf.set_params(best[1])
+    inf.attach(problem, val[1] if val is not None else data[-1][0])
+    info = dict(fields=len(data), pairs=2 * len(data), pair_tol=pair_tol, pair_seconds=t_pairs,
+                train_seconds=time.perf_counter() - t1, steps=done, best_step=best[2],
+                validation_iterations=best[0] if val is not None else None, levels=L, channels=list(ch),
+                final_loss=float(np.mean(history[-50:])), n_params=int(inf.n_params))
+    return inf, info
+
+
 def solver_pairs(problem, vals, rhs, checkpoints=(0, 4, 16, 48, 128, 320, 800, 2000), tol=1e-10):
     """(r_k, e_k) pairs from Jacobi-PCG iterates of A x = b: e_k = x* - x_k, r_k = b - A x_k.
     Restarted PCG runs with maxiter = k reproduce the iterates x_k of the full solve."""

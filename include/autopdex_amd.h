@@ -300,7 +300,7 @@ int apx_version(void);
  * (measurement of this implementation).                                      */
 int apx_set_timing(apx_plan* plan, int enabled);
 int apx_get_timing(apx_plan* plan, const char* name, double* avg_ms, int64_t* count, int64_t* bytes);
-/* Comma-separated kernel names timed per launch at timing level 1.          */
+/* Kernel names (separated by ';') timed per launch at timing level 1.      */
 int apx_set_timing_filter(apx_plan* plan, const char* names);
 /* Timer `index` (0-based, name order): name, average ms per launch, launches,
  * algorithmic bytes and flops per launch; APX_ERR_ARG past the last timer.   */

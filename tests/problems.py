@@ -72,6 +72,24 @@ def poisson_rect(nx, ny, verts=((0.0, 0.0), (2.0, 0.0), (2.0, 0.7), (0.0, 1.0)))
     return np.zeros(mask.shape), settings, static([pot]), oracle
 
 
+def poisson_vark_trapezoid(n, seed=0):
+    """poisson_vark on a trapezoid: no element is a parallelogram."""
+    return poisson_vark(n, seed, verts=((0.0, 0.0), (2.0, 0.0), (2.0, 0.7), (0.0, 1.0)))
+
+
+def poisson_kinked(n):
+    """Poisson with source on the unit square with a few interior nodes moved off the grid: the
+    grid mixes parallelograms with general quadrilaterals."""
+    dofs, settings, st, oracle = poisson_source(n, dict_dofs=False)
+    coords = settings["node coordinates"].copy()
+    h = 1.0 / n
+    for k, (i, j) in enumerate(((2, 3), (n // 2, n // 3), (n - 3, n - 2))):
+        coords[i * (n + 1) + j] += h * np.array([0.2 - 0.1 * k, 0.15])
+    settings["node coordinates"] = coords
+    oracle["coords"] = coords
+    return dofs, settings, st, oracle
+
+
 def grf(n_nodes_1d, seed, length=0.05, sigma=1.0):
     """Squared-exponential Gaussian random field on an (m x m) node grid of [0,1]^2
     by circulant embedding (periodic FFT on a doubled grid), seeded PCG64."""
@@ -87,10 +105,12 @@ def grf(n_nodes_1d, seed, length=0.05, sigma=1.0):
     return f.real[:m, :m].ravel()
 
 
-def poisson_vark(n, seed=0):
-    """Config-4 family: -div(k grad u) = f, k = exp(g), g GRF, f ~ N(0,1) per node."""
-    coords, elems = mesher.structured_mesh((n, n), UNIT_SQUARE, "quad")
-    mask = geometry.in_sdfs(coords, lambda x: geometry.psdf_polygon(x, UNIT_SQUARE))
+def poisson_vark(n, seed=0, verts=None):
+    """Config-4 family: -div(k grad u) = f, k = exp(g), g GRF, f ~ N(0,1) per node (on the unit
+    square, or on the quadrilateral verts)."""
+    verts = UNIT_SQUARE if verts is None else [list(v) for v in verts]
+    coords, elems = mesher.structured_mesh((n, n), verts, "quad")
+    mask = geometry.in_sdfs(coords, lambda x: geometry.psdf_polygon(x, verts))
     g = grf(n + 1, seed)
     f = np.random.Generator(np.random.PCG64(seed + 1_000_000)).standard_normal(coords.shape[0])
     pot = models.mixed_reference_domain_potential(
@@ -121,9 +141,10 @@ def neo_hooke_cube(n, lam=10.0, mu=1.0, disp=0.1):
     return np.zeros(mask.shape), settings, static([pot]), oracle
 
 
-def tapered_panel(n, lam=40.0, mu=0.4, total_traction=1.0, multiplier=1.0):
-    """Config-5 family: tapered panel n x n Q1, x=0 clamped, vertical traction on x=48."""
-    coords, elems = mesher.structured_mesh((n, n), TAPERED, "quad")
+def tapered_panel(n, lam=40.0, mu=0.4, total_traction=1.0, multiplier=1.0, verts=None):
+    """Config-5 family: tapered panel n x n Q1, x=0 clamped, vertical traction on x=48
+    (verts: another quadrilateral with its right edge at x = 48 and length 16)."""
+    coords, elems = mesher.structured_mesh((n, n), TAPERED if verts is None else verts, "quad")
     left = geometry.in_planes(coords, [0, 0], [1, 0])
     mask = utility.dof_select(left, [True, True])
     lines = mesher.boundary_lines((n, n), "right")

@@ -40,11 +40,15 @@ CASES = [
     ("neo_hooke_plane_strain_tiles", P.tapered_panel, (66,)),
     # 'user element' mode: Q2 Neo-Hooke and 3-node traction lines on an unstructured mesh
     ("cook_membrane_q2", P.cook_membrane_q2, ()),
+    # owner tiles on grids that are not all parallelograms: the general per-point element path
+    ("poisson_vark_trapezoid_tiles", P.poisson_vark_trapezoid, (70, 7)),
+    ("poisson_kinked_tiles", P.poisson_kinked, (70,)),
 ]
 # bit 0: owner-tile residual kernels in use; bit 1: owner-tile tangent too
 STRUCTURED = {"poisson_q1": 3, "poisson_vark_q1": 3, "neo_hooke_hex8": 0, "neo_hooke_plane_strain_traction": 1,
               "poisson_q1_tiles": 3, "poisson_rect_tiles": 3, "poisson_rect_tiles_t": 3, "poisson_vark_q1_tiles": 3,
-              "neo_hooke_plane_strain_tiles": 1, "cook_membrane_q2": 0}
+              "neo_hooke_plane_strain_tiles": 1, "cook_membrane_q2": 0, "poisson_vark_trapezoid_tiles": 3,
+              "poisson_kinked_tiles": 3}
 
 
 @pytest.mark.parametrize("name,builder,args", CASES, ids=[c[0] for c in CASES])
