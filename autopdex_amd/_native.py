@@ -9,7 +9,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libautopdex_amd.so")
 
-APX_FP64, APX_FP32 = 0, 1
+APX_FP64, APX_FP32, APX_FP64_ROUNDED = 0, 1, 2
 APX_PRECOND_JACOBI, APX_PRECOND_NONE, APX_PRECOND_LEARNED = 0, 1, 2
 MODEL_POISSON_SOURCE, MODEL_POISSON_VARK, MODEL_NEO_HOOKE, MODEL_LINE_TRACTION = 1, 2, 3, 4
 

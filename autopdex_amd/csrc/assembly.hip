@@ -278,6 +278,17 @@ void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipSt
   }
 }
 
+void assemble_tangent_rounded(Plan& P, const double* u, float* vals, hipStream_t st) {
+  if (tangent_by_tiles(P)) {
+    for (auto& S : P.sets)
+      if (S.model != APX_MODEL_LINE_TRACTION) tile_tangent(P, S, u, vals, kFP64RoundedToFP32, st);
+    return;
+  }
+  if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
+  assemble_tangent(P, u, P.vals64.p, APX_FP64, st);
+  round_to_f32(P.vals64.p, vals, P.sell_total, st);
+}
+
 int64_t tangent_coo_entries(const Plan& P) {
   int64_t n = 0;
   for (auto& S : P.sets) n += S.n_elems * ndofs_elem(P, S) * ndofs_elem(P, S);

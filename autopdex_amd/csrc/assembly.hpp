@@ -11,6 +11,11 @@ void round_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 
 void assemble_residual(Plan& P, const double* u, double* r, hipStream_t st);
 void assemble_tangent(Plan& P, const double* u, void* vals, int precision, hipStream_t st);
+// precision code of "FP64 element arithmetic, values rounded once to FP32"
+constexpr int kFP64RoundedToFP32 = APX_FP64_ROUNDED;
+// The FP64 tangent rounded once to FP32 into vals (FP32, SELL order): written directly by the
+// owner-tile kernels, else assembled in FP64 into the plan's scratch values and rounded.
+void assemble_tangent_rounded(Plan& P, const double* u, float* vals, hipStream_t st);
 void assemble_diagonal(Plan& P, const double* u, double* diag, hipStream_t st);
 // full tangent as BCOO with duplicates (data only; the reference's element-major order)
 int64_t tangent_coo_entries(const Plan& P);

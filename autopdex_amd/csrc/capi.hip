@@ -171,12 +171,18 @@ int apx_assemble_tangent(apx_plan* plan, const double* u_dev, void* vals_dev, in
   return guarded([&] {
     Plan& P = ready(plan);
     APX_REQUIRE(u_dev && vals_dev, APX_ERR_ARG, "null device pointer");
-    APX_REQUIRE(precision == APX_FP64 || precision == APX_FP32, APX_ERR_ARG, "precision must be FP64 or FP32");
-    ScopedTimer t(P, precision == APX_FP64 ? "tangent_call_f64" : "tangent_call_f32", S(stream),
-                  tangent_bytes(P, precision));
-    void* sell = sell_buffer(P, precision);
-    assemble_tangent(P, u_dev, sell, precision, S(stream));
-    sell_to_csr(P, sell, vals_dev, precision, S(stream));
+    APX_REQUIRE(precision == APX_FP64 || precision == APX_FP32 || precision == APX_FP64_ROUNDED, APX_ERR_ARG,
+                "precision must be FP64, FP32 or FP64_ROUNDED");
+    const int out = precision == APX_FP64 ? APX_FP64 : APX_FP32;
+    ScopedTimer t(P, precision == APX_FP64 ? "tangent_call_f64" : precision == APX_FP32 ? "tangent_call_f32"
+                                                                                        : "tangent_call_f64_to_f32",
+                  S(stream), tangent_bytes(P, out));
+    void* sell = sell_buffer(P, out);
+    if (precision == APX_FP64_ROUNDED)
+      assemble_tangent_rounded(P, u_dev, (float*)sell, S(stream));
+    else
+      assemble_tangent(P, u_dev, sell, precision, S(stream));
+    sell_to_csr(P, sell, vals_dev, out, S(stream));
   });
 }
 

@@ -56,12 +56,13 @@ struct NewtonCtx {
     s.seconds_residual += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return v;
   }
-  // The FP32 Krylov matrix is the FP64 tangent rounded once.  Assembled directly in FP32, the
-  // element sums carry accumulation errors of the same size as rounding but without its
-  // structure (row sums, symmetry).  On the 2048^2 Poisson problem flexible CG with the network
-  // needed 76 iterations over 6 Newton steps on the FP32-assembled matrix and 50 over 5 on the
-  // rounded one (Jacobi-PCG: 20692 and 13456); one FP64 assembly plus the rounding cost far less
-  // than one network application.
+  // The FP32 Krylov matrix is by default the FP64 tangent rounded once (the owner-tile kernels
+  // store the rounded values directly).  Assembled with FP32 element arithmetic through the
+  // general per-point path, the element sums carry accumulation errors of the size of rounding but
+  // without its structure (row sums, symmetry): on the 2048^2 Poisson problem flexible CG with
+  // the network needed 76 iterations over 6 Newton steps on such a matrix and 50 over 5 on the
+  // rounded one.  The parallelogram path's FP32 arithmetic (constant tables) keeps the structure
+  // and needs the same 50.
   bool fp32_arith = false;  // FP32 tangent: element arithmetic in FP32 instead of rounding the FP64 one
   // Tangent values of type M into vals and the preconditioner's inverse diagonal of type V
   // (an FP32 matrix may run with FP64 Krylov vectors: its diagonal is widened exactly).
@@ -71,9 +72,7 @@ struct NewtonCtx {
     auto t0 = std::chrono::steady_clock::now();
     EventPair e = timing_begin(P, st);
     if (prec == APX_FP32 && !fp32_arith) {
-      if (P.vals64.n < (size_t)P.sell_total) P.vals64.alloc(P.sell_total);
-      assemble_tangent(P, u, P.vals64.p, APX_FP64, st);
-      round_to_f32(P.vals64.p, (float*)vals, P.sell_total, st);
+      assemble_tangent_rounded(P, u, (float*)vals, st);
     } else {
       assemble_tangent(P, u, vals, prec, st);
     }
@@ -98,7 +97,9 @@ struct NewtonCtx {
     else if (fp32_arith)
       timing_collect(P, "tangent_f32", ev_tan, tangent_bytes(P, APX_FP32), ev_tan.size());
     else
-      timing_collect(P, "tangent_f64_to_f32", ev_tan, tangent_bytes(P, APX_FP64) + P.sell_total * 12, ev_tan.size());
+      timing_collect(P, "tangent_f64_to_f32", ev_tan,
+                     tangent_by_tiles(P) ? tangent_bytes(P, APX_FP32) : tangent_bytes(P, APX_FP64) + P.sell_total * 12,
+                     ev_tan.size());
   }
 };
 

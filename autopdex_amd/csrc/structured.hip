@@ -96,6 +96,12 @@ __device__ inline bool q1_parallelogram(const double (*x)[2], double& det, doubl
   return true;
 }
 
+// K_ab (a <= b) of a parallelogram with a constant coefficient, in T, from the summed tables
+template <class T>
+__device__ __forceinline__ T q1_affine_entry(int a, int b, T d, T b00, T b01, T b11) {
+  return d * (b00 * (T)q1_S(a, b, 0, 0) + b01 * (T)q1_S(a, b, 0, 1) + b11 * (T)q1_S(a, b, 1, 1));
+}
+
 // Element stiffness of a parallelogram in T: constant coefficient (KQ = false) from the
 // summed tables, or k_q per Gauss point
 template <class T, bool KQ>
@@ -108,7 +114,7 @@ __device__ inline void q1_affine_stiffness(double det, double B00, double B01, d
     for (int b = a; b < 4; ++b) {
       T v;
       if constexpr (!KQ) {
-        v = d * (b00 * (T)q1_S(a, b, 0, 0) + b01 * (T)q1_S(a, b, 0, 1) + b11 * (T)q1_S(a, b, 1, 1));
+        v = q1_affine_entry<T>(a, b, d, b00, b01, b11);
       } else {
         v = 0;
 #pragma unroll
@@ -126,7 +132,7 @@ __device__ inline void q1_affine_stiffness(double det, double B00, double B01, d
 
 template <class M>
 struct AffineModel {  // which models take the parallelogram path, and how
-  static constexpr bool ok = false;
+  static constexpr bool ok = false, nodal = false;
 };
 template <>
 struct AffineModel<PoissonSource<4, 2>> {
@@ -137,10 +143,11 @@ struct AffineModel<PoissonVarK<4, 2>> {
   static constexpr bool ok = true, nodal = true;
 };
 
+// Node, DOF and tile indices are 32-bit: tile_supported admits grids whose DOF count fits.
 struct TileArgs {
-  const double* coords;  // SoA, 2 x n_nodes
-  int64_t n_nodes;
-  int64_t nx, ny;        // elements per direction
+  const double* cx;      // node coordinates x (SoA)
+  const double* cy;      // node coordinates y
+  int nx, ny;            // elements per direction
   const double* u;       // n_dofs
   ModelParams mp;
   const double* load;    // optional: u-independent residual part (n_dofs), added to r
@@ -151,7 +158,7 @@ struct TileArgs {
   void* vals;            // SELL values (tangent)
   int overwrite;         // 1: r = sum, 0: r += sum
   int want_source;       // residual: evaluate the u-independent part in the kernel
-  int64_t tiles, tiles_j;  // tiles in the grid, tiles per tile row
+  int tiles, tiles_j;    // tiles in the grid, tiles per tile row
 };
 
 // index of K_ab (a <= b, or swapped) in the upper triangle of a symmetric 4 x 4 matrix
@@ -176,7 +183,9 @@ __device__ inline void store_matrix(T* er, int el, const T (&K)[4][4]) {
 // AFF: every element of the grid is a parallelogram (checked once per plan by
 // k_parallelogram_check with the same predicate), so only the parallelogram path is compiled;
 // the general per-point path holds registers that cost the kernel occupancy.
-template <class M, class T, int MODE, bool AFF>
+// TO: type of the stored tangent values (FP32 for the FP64 tangent rounded once, TO = float
+// with T = double: the same bits as rounding the FP64 values afterwards)
+template <class M, class T, int MODE, bool AFF, class TO = T>
 __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   constexpr int D = M::D, NPE = 4;
   static_assert(M::NPE == 4 && M::DIM == 2, "owner tiles are for Q1 quads");
@@ -191,7 +200,11 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   // Element matrices of the tile models are symmetric, bit for bit (K_ab and K_ba are the same
   // products summed in the same order), so the tangent keeps a <= b only and the diagonal
   // keeps K_aa only.
-  constexpr int NE_ = MODE == TILE_RES ? NPE * D : MODE == TILE_TAN ? NPE * (NPE + 1) / 2 : NPE;
+  // GEOM: a constant-coefficient parallelogram tangent keeps only (det J, B00, B01, B11) per
+  // element, and each owner forms the entries of its row from them (the same expression, so the
+  // same bits): 4 planes instead of 10
+  constexpr bool GEOM = AFF && MODE == TILE_TAN && !AffineModel<M>::nodal;
+  constexpr int NE_ = MODE == TILE_RES ? NPE * D : GEOM ? 4 : MODE == TILE_TAN ? NPE * (NPE + 1) / 2 : NPE;
   // one plane of kNE words per element entry: entry e of element el sits at er[e * kNE + el]
   // (e = a * D + c for residual rows, sym_entry(a, b) for matrices, a for diagonals), so the
   // lanes of a wave touch consecutive words in both phases and the LDS banks do not conflict
@@ -199,17 +212,17 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
 
   // XCD-aware tile order (1D grid rounded up to a multiple of 8): consecutive logical tiles,
   // row-major over the tile grid, share an L2
-  const int64_t lin = xcd_block(blockIdx.x, gridDim.x);
+  const int lin = (int)xcd_block(blockIdx.x, gridDim.x);
   if (lin >= A.tiles) return;  // whole workgroup: no barrier is left waiting
-  const int64_t i0 = (lin / A.tiles_j) * kTI, j0 = (lin % A.tiles_j) * kTJ;
-  const int64_t nx = A.nx, ny = A.ny;
+  const int i0 = (lin / A.tiles_j) * kTI, j0 = (lin % A.tiles_j) * kTJ;
+  const int nx = A.nx, ny = A.ny;
   const int tid = threadIdx.x;
   // owner-phase inputs first, so that their latency overlaps the staging loads and the
   // element phase: thread tid < 441 owns node (i0 + tid / 63, j0 + tid % 63)
   const int oli = tid / kTJ, olj = tid % kTJ;
-  const int64_t oI = i0 + oli, oJ = j0 + olj;
+  const int oI = i0 + oli, oJ = j0 + olj;
   const bool owner = tid < kTI * kTJ && oI <= nx && oJ <= ny;
-  const int64_t onode = owner ? oI * (ny + 1) + oJ : 0;
+  const int onode = owner ? oI * (ny + 1) + oJ : 0;
   double pl[D], pr[D];  // the node's load and previous r (residual), previous value (diagonal)
 #pragma unroll
   for (int c = 0; c < D; ++c) pl[c] = pr[c] = 0.0;
@@ -234,11 +247,11 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   for (int q = 0; q < kSR; ++q) {
     const int idx = tid + q * kTBS;
     const int li = idx / kNJ, lj = idx % kNJ;
-    const int64_t I = i0 - 1 + li, J = j0 - 1 + lj;
+    const int I = i0 - 1 + li, J = j0 - 1 + lj;
     const bool in = idx < kNN && I >= 0 && I <= nx && J >= 0 && J <= ny;
-    const int64_t node = in ? I * (ny + 1) + J : 0;
-    gx[q][0] = in ? A.coords[node] : 0.0;
-    gx[q][1] = in ? A.coords[A.n_nodes + node] : 0.0;
+    const int node = in ? I * (ny + 1) + J : 0;
+    gx[q][0] = in ? A.cx[node] : 0.0;
+    gx[q][1] = in ? A.cy[node] : 0.0;
 #pragma unroll
     for (int c = 0; c < D; ++c) gu[q][c] = (MODE == TILE_RES && in) ? A.u[node * D + c] : 0.0;
     gn[q][0] = (M::NODAL && in) ? A.mp.nodal[0][node] : 0.0;
@@ -276,7 +289,7 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   {
     const int el = tid;
     const int lei = el / kEJ, lej = el % kEJ;
-    const int64_t ei = i0 - 1 + lei, ej = j0 - 1 + lej;
+    const int ei = i0 - 1 + lei, ej = j0 - 1 + lej;
     if (ei >= 0 && ei < nx && ej >= 0 && ej < ny) {
       double xl[NPE][2], ul[NPE][D], nl[NPE][2];
 #pragma unroll
@@ -332,6 +345,11 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
               }
               er[a * D * kNE + el] = r;
             }
+          } else if constexpr (GEOM) {
+            er[el] = (T)det;
+            er[kNE + el] = (T)B00;
+            er[2 * kNE + el] = (T)B01;
+            er[3 * kNE + el] = (T)B11;
           } else {
             T K[4][4];
             q1_affine_stiffness<T, AffineModel<M>::nodal>(det, B00, B01, B11, kq, K);
@@ -379,7 +397,7 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
   // owner phase: thread -> node (i0 + li, j0 + lj)
   if (!owner) return;
   const int li = oli, lj = olj;
-  const int64_t I = oI, J = oJ;
+  const int I = oI, J = oJ;
   // the four elements around the node, in element order, with the node's local index:
   // (I-1, J-1): 2, (I-1, J): 3, (I, J-1): 1, (I, J): 0
   const bool ok[4] = {I >= 1 && J >= 1, I >= 1 && J < ny, I < nx && J >= 1, I < nx && J < ny};
@@ -406,6 +424,16 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (!ok[k]) continue;
+      if constexpr (GEOM) {
+        const T d = er[elx[k]], b00 = er[kNE + elx[k]], b01 = er[2 * kNE + elx[k]], b11 = er[3 * kNE + elx[k]];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) {
+          const int dI = kEi[k] + kDi[b], dJ = kEjo[k] + kDj[b];
+          const int lo = kLoc[k] < b ? kLoc[k] : b, hi = kLoc[k] < b ? b : kLoc[k];
+          v[dI + 1][dJ + 1] += q1_affine_entry<T>(lo, hi, d, b00, b01, b11);
+        }
+        continue;
+      }
 #pragma unroll
       for (int b = 0; b < NPE; ++b) {
         const int dI = kEi[k] + kDi[b], dJ = kEjo[k] + kDj[b];
@@ -417,18 +445,18 @@ __global__ __launch_bounds__(kTBS) void k_tile(TileArgs A) {
       A.r[onode] = A.overwrite ? (double)v[1][1] : pr[0] + (double)v[1][1];
     } else {
       if (orow < 0) return;  // Dirichlet node: no reduced row
-      T* vals = (T*)A.vals + obase + (orow & 63);
+      TO* vals = (TO*)A.vals + obase + (orow & 63);
       int k = 0;
 #pragma unroll
       for (int dI = -1; dI <= 1; ++dI)
 #pragma unroll
         for (int dJ = -1; dJ <= 1; ++dJ) {
           if (sm[(li + 1 + dI) * kNJ + lj + 1 + dJ] == 0) {  // free node inside the grid
-            vals[(int64_t)k * 64] = v[dI + 1][dJ + 1];
+            vals[k * 64] = (TO)v[dI + 1][dJ + 1];
             ++k;
           }
         }
-      for (; k < owidth; ++k) vals[(int64_t)k * 64] = 0;  // chunk padding
+      for (; k < owidth; ++k) vals[k * 64] = 0;  // chunk padding
     }
   }
 }
@@ -470,14 +498,14 @@ bool all_parallelograms(Plan& P, ElemSet& S, hipStream_t st) {
   return S.all_parallelograms == 1;
 }
 
-template <class M, class T, int MODE>
+template <class M, class T, int MODE, class TO = T>
 void run_tile(Plan& P, ElemSet& S, const double* u, double* r, void* vals, const double* load, bool overwrite,
               hipStream_t st) {
   TileArgs A;
-  A.coords = P.coords.p;
-  A.n_nodes = P.n_nodes;
-  A.nx = S.gnx;
-  A.ny = S.gny;
+  A.cx = P.coords.p;
+  A.cy = P.coords.p + P.n_nodes;
+  A.nx = (int)S.gnx;
+  A.ny = (int)S.gny;
   A.u = u;
   A.mp = S.params;
   A.load = load;
@@ -492,10 +520,11 @@ void run_tile(Plan& P, ElemSet& S, const double* u, double* r, void* vals, const
     APX_REQUIRE(S.params.nodal[0] && S.params.nodal[1], APX_ERR_STATE,
                 "variable-coefficient Poisson set needs nodal fields g and f bound");
   }
-  A.tiles_j = (S.gny + 1 + kTJ - 1) / kTJ;
-  A.tiles = A.tiles_j * ((S.gnx + 1 + kTI - 1) / kTI);
-  const int64_t g = round_xcd(A.tiles);
+  const int64_t tiles_j = (S.gny + 1 + kTJ - 1) / kTJ, tiles = tiles_j * ((S.gnx + 1 + kTI - 1) / kTI);
+  const int64_t g = round_xcd(tiles);
   APX_REQUIRE(g < ((int64_t)1 << 31), APX_ERR_UNSUPPORTED, "structured grid too large for the owner-tile launch");
+  A.tiles_j = (int)tiles_j;
+  A.tiles = (int)tiles;
   bool aff = false;
   if constexpr (AffineModel<M>::ok) {
     // the source model's residual takes the parallelogram path only with its source part given
@@ -503,9 +532,9 @@ void run_tile(Plan& P, ElemSet& S, const double* u, double* r, void* vals, const
   }
   if (aff) {
     if constexpr (AffineModel<M>::ok)
-      hipLaunchKernelGGL((k_tile<M, T, MODE, true>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
+      hipLaunchKernelGGL((k_tile<M, T, MODE, true, TO>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
   } else {
-    hipLaunchKernelGGL((k_tile<M, T, MODE, false>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
+    hipLaunchKernelGGL((k_tile<M, T, MODE, false, TO>), dim3((unsigned)g), dim3(kTBS), 0, st, A);
   }
   APX_LAUNCH_CHECK();
 }
@@ -530,6 +559,7 @@ static const double* source_load(Plan& P, ElemSet& S, hipStream_t st) {
 
 bool tile_supported(const Plan& P, const ElemSet& S, int mode) {
   if (!S.structured) return false;
+  if (P.n_dofs >= ((int64_t)1 << 31) - 64) return false;  // the owner-tile kernels index in 32 bits
   const bool scalar = S.model == APX_MODEL_POISSON_SOURCE || S.model == APX_MODEL_POISSON_VARK;
   if (mode == 0) return scalar || S.model == APX_MODEL_NEO_HOOKE;
   return scalar && P.d == 1;
@@ -550,13 +580,17 @@ void tile_residual(Plan& P, ElemSet& S, const double* u, double* r, bool overwri
 }
 
 void tile_tangent(Plan& P, ElemSet& S, const double* u, void* vals, int precision, hipStream_t st) {
-  const bool f64 = precision == APX_FP64;
+  const bool f64 = precision == APX_FP64, rounded = precision == kFP64RoundedToFP32;
   switch (S.model) {
     case APX_MODEL_POISSON_SOURCE:
       if (f64) return run_tile<PoissonSource<4, 2>, double, TILE_TAN>(P, S, u, nullptr, vals, nullptr, true, st);
+      if (rounded)
+        return run_tile<PoissonSource<4, 2>, double, TILE_TAN, float>(P, S, u, nullptr, vals, nullptr, true, st);
       return run_tile<PoissonSource<4, 2>, float, TILE_TAN>(P, S, u, nullptr, vals, nullptr, true, st);
     case APX_MODEL_POISSON_VARK:
       if (f64) return run_tile<PoissonVarK<4, 2>, double, TILE_TAN>(P, S, u, nullptr, vals, nullptr, true, st);
+      if (rounded)
+        return run_tile<PoissonVarK<4, 2>, double, TILE_TAN, float>(P, S, u, nullptr, vals, nullptr, true, st);
       return run_tile<PoissonVarK<4, 2>, float, TILE_TAN>(P, S, u, nullptr, vals, nullptr, true, st);
     default:
       throw Error(APX_ERR_INTERNAL, "owner-tile tangent: unsupported model");

@@ -101,7 +101,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   static_assert(NT % NB == 0, "channel groups must tile the output channels");
   constexpr int CB = (NB == NT) ? CO : 16 * NB;  // output channels of this block
   constexpr int SX = (kTileH + 2) * (kTileW + 2) * S;
-  constexpr int SO = kTileH * kTileW * CB * (OUT_F32 ? 2 : 1);  // output staging (uint16 units)
+  // output staging: pixel stride CB + 8 bf16 (16 bytes of padding), so that the four lane groups
+  // of a fragment store, four pixels apart, land on different LDS banks
+  constexpr int SOS = OUT_F32 ? CB : CB + 8;
+  constexpr int SO = kTileH * kTileW * SOS * (OUT_F32 ? 2 : 1);  // output staging (uint16 units)
   constexpr int SBUF = SX > SO ? SX : SO;
   constexpr int CH8 = (CI + 7) / 8;  // 16-byte input chunks per pixel
   constexpr int NP = (kTileH + 2) * (kTileW + 2);
@@ -119,6 +122,15 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   // every load below is unconditional, at an address clamped into its array, so that all of a
   // thread's loads are in flight at once (a load under a per-lane condition makes the compiler
   // wait for it at the branch join); what lies outside is dropped or zeroed at the LDS store
+  // the biases of this lane's output channels, once per block and ahead of the weights, so that
+  // the wait for the weights covers them (a load used in the epilogue would otherwise be waited
+  // for with every tile, behind the next tile's prefetch)
+  float bias[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int cl = n * 16 + (int)(threadIdx.x & 15);
+    bias[n] = (a.bias && cl < CB && co0 + cl < CO) ? a.bias[co0 + cl] : 0.f;
+  }
   u32x4 qw[RW];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
@@ -265,7 +277,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
       for (int n = 0; n < NB; ++n) {
         const int cl = n * 16 + lr;  // channel inside the block's group
         if (cl < CB && co0 + cl < CO) {
-          const float bb = a.bias ? a.bias[co0 + cl] : 0.f;
+          const float bb = bias[n];
 #pragma unroll
           for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -276,7 +288,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
               if constexpr (OUT_F32)
                 ((float*)sbuf)[(wave * kTileW + px) * CB + cl] = v * scale;
               else
-                sbuf[(wave * kTileW + px) * CB + cl] = f2bf(v);
+                sbuf[(wave * kTileW + px) * SOS + cl] = f2bf(v);
             }
         }
       }
@@ -293,10 +305,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
         for (int i = lane; i < npx * CB; i += 64) dst[(i / CB) * CO + i % CB] = src[i];
       } else {
         constexpr int C8 = CB / 8;  // 16-byte chunks per pixel of this group
-        const uint16_t* src = sbuf + wave * kTileW * CB;
+        const uint16_t* src = sbuf + wave * kTileW * SOS;
         uint16_t* dst = (uint16_t*)a.y + row0;
         for (int i = lane; i < npx * C8; i += 64)
-          *(u32x4*)(dst + (i / C8) * CO + (i % C8) * 8) = *(const u32x4*)(src + 8 * i);
+          *(u32x4*)(dst + (i / C8) * CO + (i % C8) * 8) = *(const u32x4*)(src + (i / C8) * SOS + (i % C8) * 8);
       }
     }
   }

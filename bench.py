@@ -97,7 +97,7 @@ def options(static, method):
         if method == "mixed_fp32":
             s["hip tangent arithmetic"] = "fp32"
             s["hip krylov vectors"] = "fp32"
-    elif method in ("learned", "learned_mixed", "learned_mixed_fp32_elements"):
+    elif method in ("learned", "learned_mixed", "learned_mixed_fp32_elements", "learned_family_mixed"):
         s["type of preconditioner"] = "learned"
         s["linear tolerance"] = 1e-10
         s["linear maxiter"] = 3000
@@ -300,6 +300,61 @@ def probe(args):
     print(json.dumps({"probe": "ok", "copy_bytes": src.numel() * 8, "n_free": prob.n_free, "nnz": prob.nnz}))
 
 
+def build_cube_problem(n, lam=10.0, mu=1.0, disp=0.02):
+    """Compressible Neo-Hooke on the unit cube, n^3 Hex8: x = 0 clamped, u_x = disp on x = 1."""
+    from autopdex_amd import backend, geometry, mesher, models, seeder, spaces, utility
+    cube = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
+    coords, elems = mesher.structured_mesh((n, n, n), cube, "brick")
+    left = geometry.in_planes(coords, [0, 0, 0], [1, 0, 0])
+    right = geometry.in_planes(coords, [1, 0, 0], [1, 0, 0])
+    mask = utility.dof_select(left, [True, True, True]) | utility.dof_select(right, [True, False, False])
+    values = np.zeros(mask.shape)
+    values[right, 0] = disp
+    pot = models.mixed_reference_domain_potential(
+        models.hyperelastic_integrand(models.neo_hooke, lam, mu), {"u": spaces.fem_iso_line_quad_brick},
+        *seeder.gauss_legendre_nd(3, 2), "u")
+    static = {"assembling mode": ("user potential",), "solution structure": ("nodal imposition",),
+              "model": (pot,), "solver type": "newton", "solver backend": "hip", "solver": "cg",
+              "type of preconditioner": "jacobi", "verbose": -1}
+    settings = {"connectivity": (elems,), "dirichlet dofs": mask, "node coordinates": coords,
+                "dirichlet conditions": values}
+    return backend.get_problem(np.zeros(mask.shape), settings, static), static
+
+
+def probe_cube(args):
+    """The 64^3 Hex8 Neo-Hooke configuration's kernels (colour-pass assembly, 3-DOF SpMV), for
+    kernel-trace and counter runs: one Newton step of 20 Jacobi-PCG iterations from the first
+    load step's Dirichlet values, FP64, then the mixed method (FP32 element arithmetic, FP64
+    Krylov vectors).  Prints the plan's kernel timers (HIP events) with their algorithmic bytes."""
+    import torch
+    from autopdex_amd import _native as N
+    from autopdex_amd import solver
+    n = 64 if args.n == 2048 else args.n
+    t0 = time.perf_counter()
+    prob, static = build_cube_problem(n)
+    plan_s = time.perf_counter() - t0
+    u = torch.zeros(prob.n_dofs, dtype=torch.float64, device=prob.device)
+    dv = torch.zeros_like(u)
+    out = {"probe": "cube", "n": n, "n_dofs": prob.n_dofs, "n_free": prob.n_free, "nnz": prob.nnz,
+           "plan_build_s": plan_s, "methods": {}}
+    for name, extra in (("fp64", {}), ("mixed_fp32_elements", {"hip precision": "mixed", "hip tangent arithmetic": "fp32",
+                                                                 "reliable delta": 0.0})):
+        s = dict(static, **extra, **{"linear maxiter": 20, "linear tolerance": 0.0})
+        opts = solver.newton_options(s, newton_tol=0.0, maxiter=1)
+        N.call("apx_set_timing", prob.plan, 0)
+        run(prob, opts, u, dv, torch)  # warm-up
+        N.call("apx_set_timing", prob.plan, 1)
+        for _ in range(3):
+            run(prob, opts, u, dv, torch)
+        out["methods"][name] = kernel_table(prob)
+        N.call("apx_set_timing", prob.plan, 0)  # level 0 clears the timers
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=prob.device)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps(out))
+
+
 def build_vark_problem(n):
     """-div(exp(g) grad u) = f on the unit square, n x n Q1, psdf Dirichlet mask (corners free)."""
     from autopdex_amd import backend, geometry, mesher, models, seeder, spaces
@@ -325,7 +380,9 @@ def c4_workload(args):
     seeds 10000 .. 10000 + M - 1.  One network (g as input channel) is trained on two plain
     Jacobi-PCG (r, A^-1 r) pairs per training field; every held-out field is then solved from
     u = 0 by flexible CG with the network and by FP64 Jacobi-PCG, both to a 1e-8 relative
-    residual (the FP64 residual b - A x of each result is checked on the device)."""
+    residual (the FP64 residual b - A x of each result is checked on the device).  A flexible-CG
+    solve that stagnates is finished by Jacobi-PCG from its iterate, as the Newton driver does,
+    and both parts count for the learned method."""
     import torch
     from autopdex_amd import _native as N
     from autopdex_amd import learned
@@ -347,6 +404,8 @@ def c4_workload(args):
     net, info = learned.fit_field_family(prob, H, H, field, list(range(args.fields)), steps=args.train_steps,
                                          validation_seed=9999, log=log)
     log(f"[c4] trained: {info}")
+    if args.save_net:
+        net.save(args.save_net)
     r2f = torch.from_numpy(prob.free_dofs().astype(np.int64)).cuda()
     u0 = torch.zeros(prob.n_dofs, dtype=torch.float64, device="cuda")
     r = torch.empty_like(u0)
@@ -362,6 +421,16 @@ def c4_workload(args):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         x, kf = learned.fcg(prob, vals, b, tol=1e-8, maxiter=3000)
+        fell_back = not kf.converged
+        its_f = kf.iterations
+        if fell_back:
+            # as the Newton driver does: keep flexible CG's progress, finish with Jacobi-PCG (its
+            # time and iterations count for the learned method)
+            b2 = b - learned.spmv(prob, vals, x)
+            tol2 = 1e-8 * float(torch.linalg.norm(b)) / max(float(torch.linalg.norm(b2)), 1e-300)
+            x2, k2 = learned.pcg(prob, vals, b2, tol=min(tol2, 1.0))
+            x = x + x2
+            its_f += k2.iterations
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         xp, kp = learned.pcg(prob, vals, b, tol=1e-8)
@@ -369,9 +438,9 @@ def c4_workload(args):
         t3 = time.perf_counter()
         res_f = float(torch.linalg.norm(b - learned.spmv(prob, vals, x)) / torch.linalg.norm(b))
         res_p = float(torch.linalg.norm(b - learned.spmv(prob, vals, xp)) / torch.linalg.norm(b))
-        rows.append((kf.iterations, kp.iterations, t2 - t1, t3 - t2, res_f, res_p, bool(kf.converged)))
+        rows.append((its_f, kp.iterations, t2 - t1, t3 - t2, res_f, res_p, not fell_back))
         if (k + 1) % 20 == 0:
-            log(f"[c4] held-out {k + 1}/{args.heldout}: FCG {kf.iterations} it {t2 - t1:.3f} s, "
+            log(f"[c4] held-out {k + 1}/{args.heldout}: FCG {its_f} it {t2 - t1:.3f} s, "
                 f"PCG {kp.iterations} it {t3 - t2:.3f} s")
     a = np.array([r_[:6] for r_ in rows], dtype=np.float64)
     out = {
@@ -388,6 +457,7 @@ def c4_workload(args):
                     "fcg_seconds_median": float(np.median(a[:, 2])), "pcg_seconds_median": float(np.median(a[:, 3])),
                     "fcg_seconds_total": float(a[:, 2].sum()), "pcg_seconds_total": float(a[:, 3].sum()),
                     "fcg_converged": int(sum(r_[6] for r_ in rows)),
+                    "fallbacks_to_jacobi_pcg": int(sum(not r_[6] for r_ in rows)),
                     "fcg_true_relative_residual_max": float(a[:, 4].max()),
                     "pcg_true_relative_residual_max": float(a[:, 5].max())},
     }
@@ -407,16 +477,20 @@ def main():
     ap.add_argument("--seed", type=int, default=0, help="seed of the preconditioner's training data")
     ap.add_argument("--save-net", default=None, help="write the trained network (.npz)")
     ap.add_argument("--load-net", default=None, help="use a network written by --save-net instead of training")
+    ap.add_argument("--family-net", default=None,
+                    help="method learned_family_mixed: a variable-coefficient family network (bench.py --workload c4 "
+                         "--save-net) applied with k = 1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="bounded kernel probe for counter runs")
-    ap.add_argument("--workload", default="poisson", choices=("poisson", "c4"),
+    ap.add_argument("--workload", default="poisson", choices=("poisson", "c3", "c4"),
                     help="poisson: the 2048^2 time-to-solution benchmark; c4: the learned preconditioner on "
-                         "held-out variable-coefficient fields at 1024^2")
+                         "held-out variable-coefficient fields at 1024^2; c3 (with --probe): the 64^3 Hex8 "
+                         "Neo-Hooke kernels")
     ap.add_argument("--fields", type=int, default=2000, help="c4: training fields")
     ap.add_argument("--heldout", type=int, default=200, help="c4: held-out fields")
     args = ap.parse_args()
     if args.probe:
-        return probe(args)
+        return probe_cube(args) if args.workload == "c3" else probe(args)
     if args.workload == "c4":
         if args.n == 2048:
             args.n = 1024
@@ -452,15 +526,34 @@ def main():
         if args.save_net:
             net.save(args.save_net)
         print(f"[bench] learned preconditioner: {learned_info}", file=sys.stderr, flush=True)
+    # a network trained on the variable-coefficient family (never on this operator), applied with
+    # the log-coefficient channel g = 0, i.e. k = 1
+    family = None
+    if "learned_family_mixed" in methods:
+        from autopdex_amd import learned
+        if not args.family_net:
+            raise SystemExit("method learned_family_mixed needs --family-net (bench.py --workload c4 --save-net)")
+        fnet = learned.UNetPreconditioner.load(args.family_net)
+        family = (fnet, torch.zeros(prob.n_dofs, dtype=torch.float64, device=dev))
+        learned_info = dict(learned_info or {}, family_net=args.family_net)
+
+    def solve(m):
+        if m.startswith("learned"):
+            if m == "learned_family_mixed":
+                family[0].attach(prob, family[1])
+            else:
+                net.attach(prob)
+        return run(prob, opts[m], u, dv, torch)
+
     for w in range(args.warmup):
         for m in methods:
             if m == H or w == 0:
-                run(prob, opts[m], u, dv, torch)
+                solve(m)
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     # one more untimed solve of the headline method with every network kernel timed per launch:
     # the kernel with the largest total device time is the one the roofline reports
     N.call("apx_set_timing", prob.plan, 2)
-    run(prob, opts[H], u, dv, torch)
+    solve(H)
     profile = all_timers(prob)
     dom = dominant_kernel(profile)
     N.call("apx_set_timing", prob.plan, 0)
@@ -474,7 +567,7 @@ def main():
         for m in methods:
             if m != H and k >= args.aux_steps:
                 continue
-            t, st = run(prob, opts[m], u, dv, torch)
+            t, st = solve(m)
             times[m].append(t)
             stats[m] = st
             if m == "plain" and u_plain is None:

@@ -52,7 +52,9 @@ enum apx_model {
                                    :1723-1850 with neumann_weak :744-780)                     */
 };
 
-enum apx_precision { APX_FP64 = 0, APX_FP32 = 1 };
+/* APX_FP64_ROUNDED (tangent assembly only): FP64 element arithmetic, values rounded once to
+ * FP32 -- the FP32 matrix of the mixed method, bit for bit the FP64 values cast to float.     */
+enum apx_precision { APX_FP64 = 0, APX_FP32 = 1, APX_FP64_ROUNDED = 2 };
 enum apx_precond { APX_PRECOND_JACOBI = 0, APX_PRECOND_NONE = 1, APX_PRECOND_LEARNED = 2 };
 
 /* ---- plan (mesh + DOF layout + Dirichlet mask + CSR pattern) ------------- */
@@ -114,8 +116,9 @@ int apx_plan_free_dofs(const apx_plan* plan, int32_t* red2full_host);
 int apx_assemble_residual(apx_plan* plan, const double* u_dev, double* r_dev, void* stream);
 
 /* assembler.assemble_tangent (assembler.py:682-777, :1115-1149) followed by the
- * reduction of scipy_assembling: values of the reduced CSR (nnz) in FP64 or
- * FP32 (precision = apx_precision), pattern from apx_plan_csr_pattern.      */
+ * reduction of scipy_assembling: values of the reduced CSR (nnz) in FP64, FP32
+ * or FP32 rounded from FP64 arithmetic (precision = apx_precision), pattern
+ * from apx_plan_csr_pattern.                                                */
 int apx_assemble_tangent(apx_plan* plan, const double* u_dev, void* vals_dev, int precision, void* stream);
 
 /* assembler.assemble_tangent_diagonal (assembler.py:639-680, :1090-1113):

@@ -243,28 +243,10 @@ def init_params(levels, channels, c_in, c_out, relu, seed=0, gain=None):
 LOSS_KINDS = {"residual": 0, "energy": 1, "anorm": 2}
 
 
-def train_step(orc, theta, A, red2full, r_list, e_list=None, g_list=None, kind="anorm", w_e=0.0,
-               jacobi_skip=False, quant=True):
-    """Loss and parameter gradient of one training step of the learned preconditioner
-    (apx_unet_train_step), restated from the device's definitions:
-
-    - D = |diag A|, dsq = D^-1/2 per reduced DOF (1 where the diagonal is 0); the matrix sign
-      sA = sign(A_00) (definite matrices only; the Q1 tangents of the structured mesher are
-      negative definite);
-    - per image b: s_b = RMS(dsq r_b) rounded to FP32; the network input is dsq r_b / s_b at the
-      free DOFs of the node grid (node-major, component fastest, zero at Dirichlet DOFs), FP32
-      then bf16, plus the nodal field g_b as the last channel when the network has d + 1 inputs;
-    - z_b = dsq s_b y_b (s_b y_b formed in FP32) on the free DOFs, plus dsq^2 r_b with the Jacobi
-      skip; w = A z_b, res = r_b - w;
-    - kind 'residual': L_b = ||res||^2 / ||r_b||^2;  'energy': L_b = sA (z.w / 2 - z.r_b) / (s_b^2 n);
-      'anorm': L_b = (z_b - e_b).sA A (z_b - e_b) / e_b.sA A e_b written as
-      -sA (z_b - e_b).res / (sA e_b.r_b) (exact when A e_b = r_b);  each + w_e ||z - e||^2 / ||e||^2
-      when e is given;  loss = mean_b L_b;
-    - dL/dy_b = s_b dsq dL_b/dz_b / B, stored in bf16, then the network's backward pass.
-
-    quant=False keeps every value in FP64 (no FP32 / bf16 rounding), so the loss is a smooth
-    function of theta and its gradient can be checked by finite differences.
-    Returns (loss, grad, z_list)."""
+def precondition(orc, theta, A, red2full, r_list, g_list=None, jacobi_skip=False, quant=True):
+    """The learned preconditioner's map z = M(r) per image (apx_unet_apply; the forward half of
+    apx_unet_train_step), with the definitions listed under train_step: Jacobi scaling dsq,
+    RMS scale s_b, FP32 / bf16 rounding points.  Returns (z_list, y, dsq, scales, sA)."""
     import scipy.sparse as sp
     A = sp.csr_matrix(A)
     n = A.shape[0]
@@ -292,16 +274,52 @@ def train_step(orc, theta, A, red2full, r_list, e_list=None, g_list=None, kind="
         if orc.c_in > d:
             x0[b, :, :, d] = f32(np.asarray(g_list[b])).reshape(H, W)
     y = orc.forward(theta, x0)
-    gy = np.zeros_like(y)
-    loss = 0.0
     zs = []
     for b, r in enumerate(r_list):
-        sb = scales[b]
         yb = y[b].reshape(-1)[red2full]
-        z = f32(sb * f32(yb)) * dsq
+        z = f32(scales[b] * f32(yb)) * dsq
         if jacobi_skip:
             z = z + r * dsq * dsq
         zs.append(z)
+    return zs, y, dsq, scales, sA
+
+
+def train_step(orc, theta, A, red2full, r_list, e_list=None, g_list=None, kind="anorm", w_e=0.0,
+               jacobi_skip=False, quant=True):
+    """Loss and parameter gradient of one training step of the learned preconditioner
+    (apx_unet_train_step), restated from the device's definitions:
+
+    - D = |diag A|, dsq = D^-1/2 per reduced DOF (1 where the diagonal is 0); the matrix sign
+      sA = sign(A_00) (definite matrices only; the Q1 tangents of the structured mesher are
+      negative definite);
+    - per image b: s_b = RMS(dsq r_b) rounded to FP32; the network input is dsq r_b / s_b at the
+      free DOFs of the node grid (node-major, component fastest, zero at Dirichlet DOFs), FP32
+      then bf16, plus the nodal field g_b as the last channel when the network has d + 1 inputs;
+    - z_b = dsq s_b y_b (s_b y_b formed in FP32) on the free DOFs, plus dsq^2 r_b with the Jacobi
+      skip; w = A z_b, res = r_b - w;
+    - kind 'residual': L_b = ||res||^2 / ||r_b||^2;  'energy': L_b = sA (z.w / 2 - z.r_b) / (s_b^2 n);
+      'anorm': L_b = (z_b - e_b).sA A (z_b - e_b) / e_b.sA A e_b written as
+      -sA (z_b - e_b).res / (sA e_b.r_b) (exact when A e_b = r_b);  each + w_e ||z - e||^2 / ||e||^2
+      when e is given;  loss = mean_b L_b;
+    - dL/dy_b = s_b dsq dL_b/dz_b / B, stored in bf16, then the network's backward pass.
+
+    quant=False keeps every value in FP64 (no FP32 / bf16 rounding), so the loss is a smooth
+    function of theta and its gradient can be checked by finite differences.
+    Returns (loss, grad, z_list)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    H, W = orc.H[0], orc.W[0]
+    d = orc.c_out
+    B = len(r_list)
+    f32 = (lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)) if quant else (lambda a: np.asarray(a, dtype=np.float64))
+    zs, y, dsq, scales, sA = precondition(orc, theta, A, red2full, r_list, g_list, jacobi_skip, quant)
+    red2full = np.asarray(red2full, dtype=np.int64)
+    gy = np.zeros_like(y)
+    loss = 0.0
+    for b, r in enumerate(r_list):
+        sb = scales[b]
+        z = zs[b]
         w = A @ z
         res = r - w
         e = None if e_list is None else e_list[b]

@@ -79,6 +79,11 @@ def test_element_assembly_matches_oracle(torch_gpu, name, builder, args):
     v32 = torch.empty(prob.nnz, dtype=torch.float32, device="cuda")
     N.call("apx_assemble_tangent", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(v32.data_ptr()), N.APX_FP32, st)
     assert _rel(v32.cpu().numpy().astype(np.float64), A_ref.data) < 1e-5
+    # the mixed method's matrix: FP64 arithmetic rounded once, bit for bit the FP64 values as float
+    vr = torch.empty(prob.nnz, dtype=torch.float32, device="cuda")
+    N.call("apx_assemble_tangent", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(vr.data_ptr()),
+           N.APX_FP64_ROUNDED, st)
+    assert np.array_equal(vr.cpu().numpy().view(np.uint32), v64.cpu().numpy().astype(np.float32).view(np.uint32))
     # diagonal (full length)
     g = torch.empty_like(ud)
     N.call("apx_assemble_diagonal", prob.plan, C.c_void_p(ud.data_ptr()), C.c_void_p(g.data_ptr()), st)

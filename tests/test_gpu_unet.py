@@ -81,9 +81,12 @@ def test_forward_and_gradients_match_oracle(torch_gpu, relu, c_in, c_out):
     worst = max(errs, key=errs.get)
     print(f"[case] forward rel L2 {e_l2:.2e}, max {e_max:.2e}; loss rel {e_loss:.2e}; "
           f"worst layer gradient {worst} {errs[worst]:.2e}", flush=True)
-    assert e_l2 < 2e-2 and e_max < 5e-2
-    assert e_loss < 2e-2
-    assert errs[worst] < 5e-2, (worst, errs[worst])
+    # measured: forward 2.6e-4 .. 5.3e-3 (L2), 1.4e-3 .. 1.4e-2 (max); loss 1.3e-6 .. 1.4e-4;
+    # worst layer gradient 1.4e-3 .. 3.2e-3 -- bf16 storage rounds differently where FP32 and FP64
+    # accumulation orders straddle a rounding boundary; a wrong tap, channel or tile index is O(1)
+    assert e_l2 < 1.5e-2 and e_max < 4e-2
+    assert e_loss < 1e-3
+    assert errs[worst] < 1e-2, (worst, errs[worst])
 
 
 def _tail_level(H, W, L, ch):
