@@ -16,6 +16,10 @@ each method:
                   Newton residual
   learned_mixed_fp32_elements = learned_mixed with the tangent's element
                   arithmetic in FP32 (FP32 accumulation) instead of rounding
+  learned_family_mixed = learned_mixed with a network trained on the
+                  variable-coefficient family (--family-net, written by
+                  --workload c4 --n 2048 --save-net), applied with k = 1: a
+                  network that never saw this operator (not in the default list)
 The network is trained on the GPU before the timed steps (learned.fit: pairs
 from plain Jacobi-PCG solves of random right-hand sides, A-norm loss); its
 setup time is reported in `learned_setup` (with the time-to-solution including
@@ -38,6 +42,10 @@ fixed number of scipy Jacobi-PCG iterations on the full-size reduced matrix
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--aux-steps 3] [--n 2048]
                 [--methods plain,mixed,mixed_fp32,learned,learned_mixed,learned_mixed_fp32_elements]
                 [--headline learned_mixed] [--train-steps 1500] [--no-cpu]
+python bench.py --workload c4 [--fields 2000] [--heldout 200] [--train-steps 1500]
+                (the learned preconditioner on held-out variable-coefficient fields at 1024^2)
+python bench.py --probe [--workload c3]
+                (bounded kernel probes for rocprofv3 counter runs: 2048^2 Poisson, 64^3 Hex8)
 """
 import argparse
 import ctypes as C
@@ -318,23 +326,34 @@ def build_cube_problem(n, lam=10.0, mu=1.0, disp=0.02):
               "type of preconditioner": "jacobi", "verbose": -1}
     settings = {"connectivity": (elems,), "dirichlet dofs": mask, "node coordinates": coords,
                 "dirichlet conditions": values}
-    return backend.get_problem(np.zeros(mask.shape), settings, static), static
+    return backend.get_problem(np.zeros(mask.shape), settings, static), static, values
 
 
 def probe_cube(args):
     """The 64^3 Hex8 Neo-Hooke configuration's kernels (colour-pass assembly, 3-DOF SpMV), for
-    kernel-trace and counter runs: one Newton step of 20 Jacobi-PCG iterations from the first
-    load step's Dirichlet values, FP64, then the mixed method (FP32 element arithmetic, FP64
-    Krylov vectors).  Prints the plan's kernel timers (HIP events) with their algorithmic bytes."""
+    kernel-trace and counter runs: one Newton iteration of 20 Jacobi-PCG iterations from the
+    first load step's Dirichlet values (imposed in u before the call, so the first residual is
+    the loaded one), FP64, then the mixed method (FP32 element arithmetic, FP64 Krylov vectors).
+    Prints the plan's kernel timers (HIP events) with their algorithmic bytes."""
     import torch
     from autopdex_amd import _native as N
     from autopdex_amd import solver
-    n = 64 if args.n == 2048 else args.n
+    n = args.n
     t0 = time.perf_counter()
-    prob, static = build_cube_problem(n)
+    prob, static, values = build_cube_problem(n)
     plan_s = time.perf_counter() - t0
-    u = torch.zeros(prob.n_dofs, dtype=torch.float64, device=prob.device)
+    u0 = torch.from_numpy(values.ravel().copy()).to(prob.device)
+    u = torch.zeros_like(u0)
     dv = torch.zeros_like(u)
+
+    def newton(opts):
+        from autopdex_amd.backend import current_stream
+        u.copy_(u0)
+        st = N.NewtonStats()
+        N.call("apx_newton", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(dv.data_ptr()), C.byref(opts),
+               C.byref(st), current_stream())
+        torch.cuda.synchronize()
+        return st
     out = {"probe": "cube", "n": n, "n_dofs": prob.n_dofs, "n_free": prob.n_free, "nnz": prob.nnz,
            "plan_build_s": plan_s, "methods": {}}
     for name, extra in (("fp64", {}), ("mixed_fp32_elements", {"hip precision": "mixed", "hip tangent arithmetic": "fp32",
@@ -342,11 +361,12 @@ def probe_cube(args):
         s = dict(static, **extra, **{"linear maxiter": 20, "linear tolerance": 0.0})
         opts = solver.newton_options(s, newton_tol=0.0, maxiter=1)
         N.call("apx_set_timing", prob.plan, 0)
-        run(prob, opts, u, dv, torch)  # warm-up
+        newton(opts)  # warm-up
         N.call("apx_set_timing", prob.plan, 1)
         for _ in range(3):
-            run(prob, opts, u, dv, torch)
+            st = newton(opts)
         out["methods"][name] = kernel_table(prob)
+        out["methods"][name]["krylov_iterations_per_call"] = st.krylov_iterations
         N.call("apx_set_timing", prob.plan, 0)  # level 0 clears the timers
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=prob.device)
     dst = torch.empty_like(src)
@@ -470,7 +490,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--aux-steps", type=int, default=3, help="timed steps of the methods other than the headline")
-    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--n", type=int, default=None,
+                    help="elements per direction (default: 2048; 1024 for --workload c4; 64 for the c3 probe)")
     ap.add_argument("--methods", default="plain,mixed,mixed_fp32,learned,learned_mixed,learned_mixed_fp32_elements")
     ap.add_argument("--headline", default="learned_mixed")
     ap.add_argument("--train-steps", type=int, default=1500)
@@ -489,11 +510,11 @@ def main():
     ap.add_argument("--fields", type=int, default=2000, help="c4: training fields")
     ap.add_argument("--heldout", type=int, default=200, help="c4: held-out fields")
     args = ap.parse_args()
+    if args.n is None:
+        args.n = {"c3": 64, "c4": 1024}.get(args.workload, 2048)
     if args.probe:
         return probe_cube(args) if args.workload == "c3" else probe(args)
     if args.workload == "c4":
-        if args.n == 2048:
-            args.n = 1024
         return c4_workload(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the assemble-and-solve path runs on a single MI355X "

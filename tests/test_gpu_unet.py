@@ -187,6 +187,47 @@ def test_fused_inference_apply_is_bit_identical(torch_gpu, n, relu, c_in):
     assert np.array_equal(zs[True].view(np.uint64), zs[False].view(np.uint64))
 
 
+def test_apply_at_1025_matches_oracle(torch_gpu):
+    """apx_unet_apply at 1025 x 1025 (10 levels: the weight-stationary level-0/1 kernels, the
+    fused input layer, the 64-channel implicit GEMM with 4 and with 1 channel tiles per block, the
+    grid transfers and the fused coarse levels) against the oracle's preconditioner map
+    (oracle/unet.py precondition, with the Jacobi scaling of the oracle's own tangent).
+    Measured: 9.4e-3 relative L2, 3.0e-2 of the largest value pointwise (bf16 storage rounding
+    where FP32 and FP64 accumulation orders straddle a boundary, over ten levels)."""
+    torch = torch_gpu
+    from autopdex_amd import learned
+    from autopdex_amd.learned import UNetPreconditioner, default_levels
+    from oracle import assembly as OA
+    n = 1024
+    _, _, _, o, prob = _poisson_problem(n)
+    L = default_levels(n + 1, n + 1)
+    ch = ((16, 32) + (64,) * 16)[:L]
+    print(f"[case] apply vs oracle n={n} L={L}", flush=True)
+    net = UNetPreconditioner(n + 1, n + 1, L, ch, 1, 1, relu=False, batch=1, seed=9)
+    rng = np.random.default_rng(5)
+    theta = net.get_params()
+    for ly in net.layers:
+        theta[ly["b_off"]:ly["b_off"] + ly["co"]] = 0.05 * rng.standard_normal(ly["co"])
+    net.set_params(theta)
+    net.attach(prob)
+    r = rng.standard_normal(prob.n_free)
+    rd = torch.from_numpy(r).cuda()
+    # the network's Jacobi scaling D = |diag A| comes from the matrix of the latest flexible-CG
+    # solve: one iteration sets it
+    learned.fcg(prob, learned.tangent(prob), rd, tol=0.0, maxiter=1)
+    z = net.apply(prob, rd).cpu().numpy()
+    free = ~o["mask"]
+    A = OA.reduced_tangent(o["sets"], o["coords"], np.zeros(free.size), 1, free)
+    orc = OU.UNetOracle(L, ch, 1, 1, False, n + 1, n + 1, bf16_storage=True)
+    assert orc.n_params == net.n_params
+    zs, *_ = OU.precondition(orc, theta.astype(np.float64), A, prob.free_dofs().astype(np.int64), [r])
+    z_ref = zs[0]
+    e_l2 = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
+    e_max = np.abs(z - z_ref).max() / np.abs(z_ref).max()
+    print(f"[case] apply rel L2 {e_l2:.2e}, max {e_max:.2e}", flush=True)
+    assert e_l2 < 2.5e-2 and e_max < 7e-2
+
+
 def _poisson_problem(n):
     from autopdex_amd import backend
     dofs, settings, static, o = P.poisson_source(n, dict_dofs=False)
