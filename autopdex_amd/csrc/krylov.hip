@@ -106,20 +106,22 @@ __device__ inline void finalize_sums(const double* partials, int nb, double (&su
 // Row gather of one SELL chunk: sum_k vals[k] * f(cols[k]) for this lane's row,
 // values of type M, accumulated in V (FP32 values with FP64 vectors: each
 // value is widened exactly and the products and sums are FP64).
-template <class M, class V = M, class F>
+// U entries per step: all their column and value loads are issued before the gathers (U = 8
+// for the wide rows of 3-DOF elasticity, 4 otherwise); the sum order is k = 0, 1, 2, ... either way.
+template <class M, class V = M, int U = 4, class F>
 __device__ __forceinline__ V sell_row(const M* __restrict__ v, const int32_t* __restrict__ ci, int w, F f) {
   V s = 0;
   int k = 0;
-  for (; k + 4 <= w; k += 4) {
+  for (; k + U <= w; k += U) {
     // values and columns are read once per SpMV: non-temporal, so L2 keeps the gathered vector
-    const int32_t c0 = __builtin_nontemporal_load(ci + (k + 0) * 64), c1 = __builtin_nontemporal_load(ci + (k + 1) * 64),
-                  c2 = __builtin_nontemporal_load(ci + (k + 2) * 64), c3 = __builtin_nontemporal_load(ci + (k + 3) * 64);
-    const M v0 = __builtin_nontemporal_load(v + (k + 0) * 64), v1 = __builtin_nontemporal_load(v + (k + 1) * 64),
-            v2 = __builtin_nontemporal_load(v + (k + 2) * 64), v3 = __builtin_nontemporal_load(v + (k + 3) * 64);
-    s += (V)v0 * f(c0);
-    s += (V)v1 * f(c1);
-    s += (V)v2 * f(c2);
-    s += (V)v3 * f(c3);
+    int32_t c[U];
+    M vv[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) c[j] = __builtin_nontemporal_load(ci + (k + j) * 64);
+#pragma unroll
+    for (int j = 0; j < U; ++j) vv[j] = __builtin_nontemporal_load(v + (k + j) * 64);
+#pragma unroll
+    for (int j = 0; j < U; ++j) s += (V)vv[j] * f(c[j]);
   }
   for (; k < w; ++k) s += (V)__builtin_nontemporal_load(v + k * 64) * f(__builtin_nontemporal_load(ci + k * 64));
   return s;
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(kFinBS) void k_pcg_init_fin(KState* S, const double
   }
 }
 
-template <class M, class T>
+template <class M, class T, int U = 4>
 __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
                                                       const int32_t* __restrict__ cols, const M* __restrict__ vals,
                                                       const T* __restrict__ z, T* __restrict__ pa, T* __restrict__ pb,
@@ -284,8 +286,8 @@ __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunk
   double v[1] = {0.0};
   for (int64_t c = blockIdx.x * (int64_t)wpb + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * wpb) {
     const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
-    const T s = sell_row<M, T>(vals + off + lane, cols + off + lane, w,
-                               [&](int32_t col) { return z[col] + beta * po[col]; });
+    const T s = sell_row<M, T, U>(vals + off + lane, cols + off + lane, w,
+                                  [&](int32_t col) { return z[col] + beta * po[col]; });
     const int64_t row = c * 64 + lane;
     if (row < n) {
       const T pi = z[row] + beta * po[row];
@@ -437,8 +439,12 @@ void PcgRun<M, T>::launch_iteration() {
   const int gs = grid_chunks(P.n_chunks), gu = grid_for(n);
   const bool timed = P.timing && !external;
   EventPair e1 = timed ? timing_begin(P, st) : EventPair{};
-  hipLaunchKernelGGL((k_pcg_spmv<M, T>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals,
-                     z, pa, pb, q, S, part);
+  if (P.sell_total >= 32 * (int64_t)P.n_free)  // rows of 32+ stored entries: deeper load batches
+    hipLaunchKernelGGL((k_pcg_spmv<M, T, 8>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
+                       vals, z, pa, pb, q, S, part);
+  else
+    hipLaunchKernelGGL((k_pcg_spmv<M, T, 4>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
+                       vals, z, pa, pb, q, S, part);
   if (timed) timing_end(P, e1, st);
   hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gs);
   EventPair e2 = timed ? timing_begin(P, st) : EventPair{};

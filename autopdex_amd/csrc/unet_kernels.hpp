@@ -75,15 +75,30 @@ struct ConvArgs {
 // 16 channels; CI < 16 -> k = t*CI + c over ceil(9 CI / 16) blocks.
 //
 // Memory path: the input tile (one-pixel halo) and the weights are staged in
-// LDS with 16-byte global loads and 16-byte LDS stores (pixel stride CI + 8
-// elements keeps rows 16-byte aligned and the 8-byte fragment reads of the 16
-// lanes of a row group on distinct banks).  The accumulators go back through
-// LDS so that every wavefront stores its output row -- 64 pixels x CO channels,
-// contiguous in NHWC -- with 16-byte stores.
+// LDS with 16-byte global loads and 16-byte LDS stores.  The accumulators go
+// back through LDS so that every wavefront stores its output row -- 64 pixels x
+// CO channels, contiguous in NHWC -- with 16-byte stores.
+//
+// LDS strides.  A 16-byte LDS read (the fragments of the 16x16x32 MFMA) is served
+// for 32 lanes at a time over 128 four-byte banks (measured on gfx950 with
+// rocprofv3's SQ_LDS_BANK_CONFLICT on lane patterns).  Lane (lr, lg), lr = 0..15,
+// lg = 0..3, reads row lr at byte offset 16 lg; lanes 0..31 (lg = 0, 1) are
+// conflict-free exactly when the row stride is 32 mod 64 bytes (it is then an odd
+// multiple of 32 bytes, so 16 lr + ... covers every 16-byte bank group once).  So
+// the pixel stride of 32- and 64-channel tiles is CI + 16 elements and the weight
+// rows of the 16x16x32 path are WK + 16 (WK a multiple of 32).  The CI + 8 stride
+// they replace (144 bytes at 64 channels) made every fragment read 2-way.  Blocks
+// with one 16-channel output tile (NB = 1) keep CI + 8 for the input tile: the wider
+// tile would take a 64-channel block past 80 KB of LDS and to one block per compute
+// unit, and on the small grids those blocks run at their latency floor.
+// Rows stay 16-byte aligned either way.  16-channel tiles (8-byte fragment reads
+// of the 16x16x16 MFMA) keep CI + 8.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int CI>
-__host__ __device__ constexpr int conv_lds_stride() { return CI >= 16 ? CI + 8 : 8; }
+template <int CI, int NB>
+__host__ __device__ constexpr int conv_lds_stride() {
+  return (CI >= 32 && NB > 1) ? CI + 16 : CI >= 16 ? CI + 8 : 8;
+}
 
 // NB: 16-wide output-channel tiles per block; blockIdx.y selects the channel
 // group (gridDim.y = NT / NB).  Small grids use NB = 1 so that more blocks
@@ -93,10 +108,10 @@ __host__ __device__ constexpr int conv_lds_stride() { return CI >= 16 ? CI + 8 :
 template <int CI, int CO, bool OUT_F32, int NB>
 __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   constexpr int COP = round16(CO);
-  constexpr int S = conv_lds_stride<CI>();
+  constexpr int S = conv_lds_stride<CI, NB>();
   constexpr int KB = CI >= 16 ? 9 * (CI / 16) : (9 * CI + 15) / 16;  // K blocks of 16
   constexpr int WK = KB * 16;                                        // packed K length (per co)
-  constexpr int SWK = WK + 8;                                        // LDS weight row stride
+  constexpr int SWK = CI % 32 == 0 ? WK + 16 : WK + 8;               // LDS weight row stride (above)
   constexpr int NT = COP / 16;
   static_assert(NT % NB == 0, "channel groups must tile the output channels");
   constexpr int CB = (NB == NT) ? CO : 16 * NB;  // output channels of this block

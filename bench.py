@@ -42,7 +42,7 @@ fixed number of scipy Jacobi-PCG iterations on the full-size reduced matrix
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--aux-steps 3] [--n 2048]
                 [--methods plain,mixed,mixed_fp32,learned,learned_mixed,learned_mixed_fp32_elements]
                 [--headline learned_mixed] [--train-steps 1500] [--no-cpu]
-python bench.py --workload c4 [--fields 2000] [--heldout 200] [--train-steps 1500]
+python bench.py --workload c4 [--fields 2000] [--heldout 200] [--train-steps 8000]
                 (the learned preconditioner on held-out variable-coefficient fields at 1024^2)
 python bench.py --probe [--workload c3]
                 (bounded kernel probes for rocprofv3 counter runs: 2048^2 Poisson, 64^3 Hex8)
@@ -66,8 +66,8 @@ HBM_PEAK = 8.0e12
 # reading of a 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000).
 # The convolution's figure is the average over the launches of one network application (two at
 # 513 x 513, four at 257 x 257), the mix its timer averages over.
-PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double, double>", 2048): 660.2e6, ("k_pcg_spmv<float, double>", 2048): 509.5e6,
-                     ("k_pcg_spmv<float, float>", 2048): 408.5e6, ("k_fcg_spmv<float>", 2048): 543.3e6,
+PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double, double, 4>", 2048): 660.2e6, ("k_pcg_spmv<float, double, 4>", 2048): 509.5e6,
+                     ("k_pcg_spmv<float, float, 4>", 2048): 408.5e6, ("k_fcg_spmv<float>", 2048): 543.3e6,
                      ("k_conv3x3<64, 64, false, 4>", 2048): 36.96e6, ("k_conv3x3_ws<16, 16, 4, 0>", 2048): 292.1e6}
 
 
@@ -153,8 +153,8 @@ def all_timers(prob):
 
 # single-kernel solver timers and the kernel each one times (rocprofv3 names)
 SOLVER_KERNELS = {"fcg_spmv_f32": "k_fcg_spmv<float>", "fcg_spmv_f64": "k_fcg_spmv<double>",
-                  "pcg_spmv_f64": "k_pcg_spmv<double, double>", "pcg_spmv_f32": "k_pcg_spmv<float, float>",
-                  "pcg_spmv_f32m_f64v": "k_pcg_spmv<float, double>", "pcg_update_f64": "k_pcg_update<double>",
+                  "pcg_spmv_f64": "k_pcg_spmv<double, double, 4>", "pcg_spmv_f32": "k_pcg_spmv<float, float, 4>",
+                  "pcg_spmv_f32m_f64v": "k_pcg_spmv<float, double, 4>", "pcg_update_f64": "k_pcg_update<double>",
                   "pcg_update_f32": "k_pcg_update<float>"}
 
 
@@ -331,9 +331,9 @@ def build_cube_problem(n, lam=10.0, mu=1.0, disp=0.02):
 
 def probe_cube(args):
     """The 64^3 Hex8 Neo-Hooke configuration's kernels (colour-pass assembly, 3-DOF SpMV), for
-    kernel-trace and counter runs: one Newton iteration of 20 Jacobi-PCG iterations from the
-    first load step's Dirichlet values (imposed in u before the call, so the first residual is
-    the loaded one), FP64, then the mixed method (FP32 element arithmetic, FP64 Krylov vectors).
+    kernel-trace and counter runs: one Newton iteration of 20 Jacobi-PCG iterations from u = 0
+    with the first load step's Dirichlet values (imposed by apx_newton before its first
+    residual), FP64, then the mixed method (FP32 element arithmetic, FP64 Krylov vectors).
     Prints the plan's kernel timers (HIP events) with their algorithmic bytes."""
     import torch
     from autopdex_amd import _native as N
@@ -342,15 +342,14 @@ def probe_cube(args):
     t0 = time.perf_counter()
     prob, static, values = build_cube_problem(n)
     plan_s = time.perf_counter() - t0
-    u0 = torch.from_numpy(values.ravel().copy()).to(prob.device)
-    u = torch.zeros_like(u0)
-    dv = torch.zeros_like(u)
+    dvals = torch.from_numpy(values.ravel().copy()).to(prob.device)  # Dirichlet values, imposed by Newton
+    u = torch.zeros_like(dvals)
 
     def newton(opts):
         from autopdex_amd.backend import current_stream
-        u.copy_(u0)
+        u.zero_()
         st = N.NewtonStats()
-        N.call("apx_newton", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(dv.data_ptr()), C.byref(opts),
+        N.call("apx_newton", prob.plan, C.c_void_p(u.data_ptr()), C.c_void_p(dvals.data_ptr()), C.byref(opts),
                C.byref(st), current_stream())
         torch.cuda.synchronize()
         return st
@@ -494,7 +493,8 @@ def main():
                     help="elements per direction (default: 2048; 1024 for --workload c4; 64 for the c3 probe)")
     ap.add_argument("--methods", default="plain,mixed,mixed_fp32,learned,learned_mixed,learned_mixed_fp32_elements")
     ap.add_argument("--headline", default="learned_mixed")
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-steps", type=int, default=None,
+                    help="training steps of the network (default: 1500; 8000 for --workload c4)")
     ap.add_argument("--seed", type=int, default=0, help="seed of the preconditioner's training data")
     ap.add_argument("--save-net", default=None, help="write the trained network (.npz)")
     ap.add_argument("--load-net", default=None, help="use a network written by --save-net instead of training")
@@ -512,6 +512,8 @@ def main():
     args = ap.parse_args()
     if args.n is None:
         args.n = {"c3": 64, "c4": 1024}.get(args.workload, 2048)
+    if args.train_steps is None:
+        args.train_steps = 8000 if args.workload == "c4" else 1500
     if args.probe:
         return probe_cube(args) if args.workload == "c3" else probe(args)
     if args.workload == "c4":
@@ -620,7 +622,8 @@ def main():
         spmv_roof = roofline(rocname, ms, by, 0, PMC_TRAFFIC_BYTES.get((rocname, args.n)))
     res = kernels.get("residual", {})
     dtypes = {"plain": "f64", "mixed": "f32+f64", "mixed_fp32": "f32+f64", "learned": "bf16+f64",
-              "learned_mixed": "bf16+f32+f64", "learned_mixed_fp32_elements": "bf16+f32+f64"}
+              "learned_mixed": "bf16+f32+f64", "learned_mixed_fp32_elements": "bf16+f32+f64",
+              "learned_family_mixed": "bf16+f32+f64"}
     if learned_info and "train_seconds" in learned_info:
         setup = learned_info["pair_seconds"] + learned_info["train_seconds"]
         learned_info["setup_seconds"] = setup
