@@ -235,10 +235,11 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
                        N.jacobi_skip ? r : nullptr, N.scale.p, q, z, S, P.partials.p);
     timing_end(P, e0, st);
     hipLaunchKernelGGL(k_fcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv);
-    EventPair e1 = timing_begin(P, st);
-    hipLaunchKernelGGL(k_fcg_spmv<T>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
-                       vals, z, pa, pb, q, r, S, P.partials.p);
-    timing_end(P, e1, st);
+    // timed: the pair takes the SpMV's own start and end timestamps
+    const EventPair e1 = P.timing ? timing_pair(P) : EventPair{};
+    t_launch_events = P.timing ? &e1 : nullptr;
+    launch_kernel(k_fcg_spmv<T>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals, z,
+                  pa, pb, q, r, S, P.partials.p);
     hipLaunchKernelGGL(k_fcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gs);
     hipLaunchKernelGGL(k_fcg_update, dim3(gv), dim3(kBS), 0, st, n, pa, pb, q, x, r, S, P.partials.p);
     hipLaunchKernelGGL(k_fcg_update_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv);

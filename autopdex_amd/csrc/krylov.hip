@@ -438,18 +438,18 @@ template <class M, class T>
 void PcgRun<M, T>::launch_iteration() {
   const int gs = grid_chunks(P.n_chunks), gu = grid_for(n);
   const bool timed = P.timing && !external;
-  EventPair e1 = timed ? timing_begin(P, st) : EventPair{};
+  // timed: the pairs take the SpMV's and the update's own start and end timestamps
+  const EventPair e1 = timed ? timing_pair(P) : EventPair{}, e2 = timed ? timing_pair(P) : EventPair{};
+  t_launch_events = timed ? &e1 : nullptr;
   if (P.sell_total >= 32 * (int64_t)P.n_free)  // rows of 32+ stored entries: deeper load batches
-    hipLaunchKernelGGL((k_pcg_spmv<M, T, 8>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
-                       vals, z, pa, pb, q, S, part);
+    launch_kernel((k_pcg_spmv<M, T, 8>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
+                  vals, z, pa, pb, q, S, part);
   else
-    hipLaunchKernelGGL((k_pcg_spmv<M, T, 4>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
-                       vals, z, pa, pb, q, S, part);
-  if (timed) timing_end(P, e1, st);
+    launch_kernel((k_pcg_spmv<M, T, 4>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
+                  vals, z, pa, pb, q, S, part);
   hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gs);
-  EventPair e2 = timed ? timing_begin(P, st) : EventPair{};
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gu), dim3(kBS), 0, st, n, inv, pa, pb, q, x, r, z, S, part);
-  if (timed) timing_end(P, e2, st);
+  t_launch_events = timed ? &e2 : nullptr;
+  launch_kernel(k_pcg_update<T>, dim3(gu), dim3(kBS), 0, st, n, inv, pa, pb, q, x, r, z, S, part);
   hipLaunchKernelGGL(k_pcg_beta_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gu);
   if (timed) {
     ev_spmv.push_back(e1);

@@ -40,6 +40,13 @@ EventPair timing_begin(Plan& P, hipStream_t st) {
   return e;
 }
 
+EventPair timing_pair(Plan& P) {
+  EventPair e;
+  e.a = pool_event(P);
+  e.b = pool_event(P);
+  return e;
+}
+
 void timing_end(Plan& P, EventPair& e, hipStream_t st) {
   if (e.b) APX_HIP(hipEventRecord(e.b, st));
 }

@@ -4,6 +4,8 @@
 #include <memory>
 #include <string>
 
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 
 namespace apx {
@@ -92,7 +94,7 @@ struct Plan {
   DevBuf<double> net_dsq;
   bool net_dsq_set = false;
   // 0 off; 1 kernel-family timers of the solvers, plus one timer per launched network kernel
-  // (name = its template, e.g. "k_conv3x3<64, 64, false, 4>") for the names in timing_filter;
+  // (name = its template, e.g. "k_conv3x3<64, 64, false, 4, 2>") for the names in timing_filter;
   // 2 every network kernel
   int timing = 0;
   std::string timing_filter;  // ';'-separated kernel names timed per launch at level 1
@@ -111,6 +113,27 @@ struct EventPair {
 };
 EventPair timing_begin(Plan& P, hipStream_t st);
 void timing_end(Plan& P, EventPair& e, hipStream_t st);
+
+// Single-kernel timing.  timing_pair() takes two events from the pool without recording them;
+// set in t_launch_events, the pair is taken by the next launch_kernel on this thread, which
+// hands it to hipExtLaunchKernel: the events then carry that dispatch's own start and end
+// timestamps, so their elapsed time is the kernel's execution alone (a pair recorded on the
+// stream around a launch also counts the gap between the previous kernel's end and this one's
+// start).
+EventPair timing_pair(Plan& P);
+inline thread_local const EventPair* t_launch_events = nullptr;
+
+template <typename... Args, typename F = void (*)(Args...)>
+inline void launch_kernel(F kernel, const dim3& grid, const dim3& block, uint32_t shmem, hipStream_t st,
+                          Args... args) {
+  const EventPair* e = t_launch_events;
+  t_launch_events = nullptr;
+  if (e)
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, e->a, e->b, 0u, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+}
+
 // Wait for the pairs, add their durations to timer `name`, recycle events.
 void timing_collect(Plan& P, const char* name, std::vector<EventPair>& pairs, int64_t bytes, size_t count_limit);
 

@@ -38,9 +38,9 @@ constexpr int64_t kSplitTiles = 128;
 // the kernel: LDS and registers), so that no block starts after the first round
 // has finished (a second partial round would run at low occupancy).
 template <class K>
-int64_t resident_blocks(K kernel) {
+int64_t resident_blocks(K kernel, int threads = kConvThreads) {
   int per_cu = 0, dev = 0, cus = 0;
-  APX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kConvThreads, 0));
+  APX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
   APX_HIP(hipGetDevice(&dev));
   APX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   return (int64_t)std::max(1, cus) * std::max(1, per_cu);
@@ -53,11 +53,16 @@ void launch_conv_t(const ConvArgs& a, hipStream_t st) {
   if (NT > 1 && tiles < kSplitTiles) {
     static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, 1>);
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap / NT));
-    hipLaunchKernelGGL((k_conv3x3<CI, CO, F32, 1>), dim3(g, NT), dim3(kConvThreads), 0, st, a);
+    launch_kernel((k_conv3x3<CI, CO, F32, 1>), dim3(g, NT), dim3(kConvThreads), 0, st, a);
+  } else if constexpr (NT == 4) {
+    // four channel tiles: two waves per tile row (WS = 2), two tiles each
+    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT, 2>, 2 * kConvThreads);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+    launch_kernel((k_conv3x3<CI, CO, F32, NT, 2>), dim3(g), dim3(2 * kConvThreads), 0, st, a);
   } else {
     static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT>);
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
-    hipLaunchKernelGGL((k_conv3x3<CI, CO, F32, NT>), dim3(g), dim3(kConvThreads), 0, st, a);
+    launch_kernel((k_conv3x3<CI, CO, F32, NT>), dim3(g), dim3(kConvThreads), 0, st, a);
   }
 }
 
@@ -70,7 +75,7 @@ void launch_conv_ws_t(const ConvArgs& a, hipStream_t st) {
               "convolution of the input layer's output: no image, or the image is too large");
   static const int64_t cap = resident_blocks(k_conv3x3_ws<CI, CO, TR, IN>);
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
-  hipLaunchKernelGGL((k_conv3x3_ws<CI, CO, TR, IN>), dim3(g), dim3(kConvThreads), 0, st, a);
+  launch_kernel((k_conv3x3_ws<CI, CO, TR, IN>), dim3(g), dim3(kConvThreads), 0, st, a);
 }
 
 // Four-row tiles: measured faster than eight-row tiles (which hold twice the accumulators and
@@ -100,9 +105,9 @@ void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
     const int64_t tiles = (int64_t)a.B * ((a.W + 63) / 64) * ((a.H + 3) / 4);
     APX_REQUIRE(tiles < INT32_MAX, APX_ERR_UNSUPPORTED, "output layer: grid too large");
     if (ci == 16 && co == 1)
-      hipLaunchKernelGGL(k_conv_out<1>, dim3((int)tiles), dim3(256), 0, st, a);
+      launch_kernel(k_conv_out<1>, dim3((int)tiles), dim3(256), 0, st, a);
     else if (ci == 16 && co == 2)
-      hipLaunchKernelGGL(k_conv_out<2>, dim3((int)tiles), dim3(256), 0, st, a);
+      launch_kernel(k_conv_out<2>, dim3((int)tiles), dim3(256), 0, st, a);
     else
       throw Error(APX_ERR_UNSUPPORTED, "FP32-output convolution only for the 16 -> 1|2 output layer");
     return;
@@ -115,9 +120,9 @@ void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
     const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
     APX_REQUIRE(tiles < INT32_MAX, APX_ERR_UNSUPPORTED, "input layer: grid too large");
     if (ci == 1)
-      hipLaunchKernelGGL(k_conv_in<1>, dim3((int)tiles), dim3(kConvThreads), 0, st, a);
+      launch_kernel(k_conv_in<1>, dim3((int)tiles), dim3(kConvThreads), 0, st, a);
     else
-      hipLaunchKernelGGL(k_conv_in<2>, dim3((int)tiles), dim3(kConvThreads), 0, st, a);
+      launch_kernel(k_conv_in<2>, dim3((int)tiles), dim3(kConvThreads), 0, st, a);
     return;
   }
 #define APX_X(A, B) \
@@ -151,7 +156,7 @@ std::string conv_kernel_name(int ci, int co, bool f32, const ConvArgs& a) {
   const int nt = round16(co) / 16;
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
   const int nb = (nt > 1 && tiles < kSplitTiles) ? 1 : nt;
-  return "k_conv3x3<" + s(ci) + ", " + s(co) + ", false, " + s(nb) + ">";
+  return "k_conv3x3<" + s(ci) + ", " + s(co) + ", false, " + s(nb) + (nb == 4 ? ", 2>" : ", 1>");
 }
 
 // ---- grid transfer ---------------------------------------------------------------------
@@ -856,11 +861,11 @@ void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const u
               "grid transfer shape");
   const dim3 grid((Wc * (C / 8) + 255) / 256, Hc, N.batch);  // one thread per coarse node and chunk
   if (C == 16)
-    hipLaunchKernelGGL(k_prolong_add<2>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
+    launch_kernel(k_prolong_add<2>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else if (C == 32)
-    hipLaunchKernelGGL(k_prolong_add<4>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
+    launch_kernel(k_prolong_add<4>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else if (C == 64)
-    hipLaunchKernelGGL(k_prolong_add<8>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
+    launch_kernel(k_prolong_add<8>, grid, dim3(256), 0, st, coarse, skip, out, H, W, Hc, Wc, N.stop);
   else
     throw Error(APX_ERR_UNSUPPORTED, "grid transfer: channels must be 16, 32 or 64");
 }
@@ -872,11 +877,11 @@ void restrict_(UNet& N, int level_fine, int C, const uint16_t* fine, uint16_t* o
               "grid transfer shape");
   const dim3 grid((Wc * (C / 8) + 255) / 256, Hc, N.batch);
   if (C == 16)
-    hipLaunchKernelGGL(k_restrict<2>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
+    launch_kernel(k_restrict<2>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
   else if (C == 32)
-    hipLaunchKernelGGL(k_restrict<4>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
+    launch_kernel(k_restrict<4>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
   else if (C == 64)
-    hipLaunchKernelGGL(k_restrict<8>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
+    launch_kernel(k_restrict<8>, grid, dim3(256), 0, st, fine, out, H, W, Hc, Wc, N.stop);
   else
     throw Error(APX_ERR_UNSUPPORTED, "grid transfer: channels must be 16, 32 or 64");
 }
@@ -954,9 +959,14 @@ void UNet::forward(hipStream_t st, bool keep_activations) {
   const size_t fwd_index = n_fwd;
   auto record = [&](const std::string& name, int64_t bytes, int64_t flops, auto&& launch) {
     if (!(rec_all || (rec_some && filter_has(timer_plan->timing_filter, name)))) return launch();
-    EventPair e = timing_begin(*timer_plan, st);
+    // the pair takes the kernel's own start and end timestamps (launch_kernel)
+    EventPair e = timing_pair(*timer_plan);
+    struct Pending {
+      ~Pending() { t_launch_events = nullptr; }
+    } pending;
+    t_launch_events = &e;
     launch();
-    timing_end(*timer_plan, e, st);
+    APX_REQUIRE(t_launch_events == nullptr, APX_ERR_STATE, "timed network launch: the kernel did not take its events");
     ev_launch[name].push_back(LaunchRec{e, fwd_index, bytes, flops});
   };
   auto conv = [&](int li, int level, const uint16_t* x, void* y) {
@@ -1000,7 +1010,7 @@ void UNet::forward(hipStream_t st, bool keep_activations) {
   }
   if (fused)
     record("k_unet_tail", 0, 0,
-           [&] { hipLaunchKernelGGL(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop); });
+           [&] { launch_kernel(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop); });
   else
     conv(L_b, L - 1, a[L - 1].p, u[L - 1].p);
   for (int l = std::min(L - 2, c); l >= 0; --l) {

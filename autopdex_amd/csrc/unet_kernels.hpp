@@ -104,9 +104,16 @@ __host__ __device__ constexpr int conv_lds_stride() {
 // group (gridDim.y = NT / NB).  Small grids use NB = 1 so that more blocks
 // share the work and each stages only its own weight rows.  A block walks its
 // tiles with the next tile's input loads in flight (registers) during the
-// current tile's matrix work.
-template <int CI, int CO, bool OUT_F32, int NB>
-__global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
+// current tile's matrix work.  WS: wavefronts per tile row (block of 256 WS
+// threads); with WS = 2 each row's NB channel tiles are split between two waves,
+// so that two waves share each SIMD and one issues MFMAs while the other waits
+// on its LDS fragments (the block's LDS, which allows one block per compute unit,
+// is the same).
+template <int CI, int CO, bool OUT_F32, int NB, int WS = 1>
+__global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
+  constexpr int BS = kConvThreads * WS;
+  constexpr int NBW = NB / WS;  // channel tiles per wavefront
+  static_assert(NB % WS == 0, "the waves of a row split its channel tiles evenly");
   constexpr int COP = round16(CO);
   constexpr int S = conv_lds_stride<CI, NB>();
   constexpr int KB = CI >= 16 ? 9 * (CI / 16) : (9 * CI + 15) / 16;  // K blocks of 16
@@ -123,7 +130,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   constexpr int SBUF = SX > SO ? SX : SO;
   constexpr int CH8 = (CI + 7) / 8;  // 16-byte input chunks per pixel
   constexpr int NP = (kTileH + 2) * (kTileW + 2);
-  constexpr int NC = NP * CH8, RC = (NC + kConvThreads - 1) / kConvThreads;
+  constexpr int NC = NP * CH8, RC = (NC + BS - 1) / BS;
   static_assert(OUT_F32 || CB % 8 == 0, "bf16 outputs are stored in 8-channel chunks");
   __shared__ __attribute__((aligned(16))) uint16_t sbuf[SBUF];
   __shared__ __attribute__((aligned(16))) uint16_t sw[16 * NB * SWK];
@@ -133,7 +140,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   // weights -> LDS once per block: global [co][WK] -> LDS [co - co0][SWK], 16-byte chunks; the
   // weight loads and the first tile's loads are all issued before the weights are stored, so
   // that the two round trips overlap (it matters on small grids, where a launch has one tile)
-  constexpr int NW = 16 * NB * (WK / 8), RW = (NW + kConvThreads - 1) / kConvThreads;
+  constexpr int NW = 16 * NB * (WK / 8), RW = (NW + BS - 1) / BS;
   // every load below is unconditional, at an address clamped into its array, so that all of a
   // thread's loads are in flight at once (a load under a per-lane condition makes the compiler
   // wait for it at the branch join); what lies outside is dropped or zeroed at the LDS store
@@ -149,13 +156,14 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   u32x4 qw[RW];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
-    const int i = min((int)threadIdx.x + r * kConvThreads, NW - 1);
+    const int i = min((int)threadIdx.x + r * BS, NW - 1);
     qw[r] = *(const u32x4*)(a.w + (int64_t)(co0 + i / (WK / 8)) * WK + (i % (WK / 8)) * 8);
   }
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + kTileH - 1) / kTileH;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = (threadIdx.x >> 6) % kTileH, part = (threadIdx.x >> 6) / kTileH, lane = threadIdx.x & 63;
   const int lr = lane & 15, lg = lane >> 4;
+  const int n0 = part * NBW;  // this wave's first channel tile
   // input tile loads with a one-pixel halo into registers; bit r of qin: chunk r lies inside
   // the image (otherwise it is stored as zeros)
   static_assert(RC <= 32, "one validity bit per staged chunk");
@@ -171,7 +179,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
     qin = 0;
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      const int i = threadIdx.x + r * kConvThreads;
+      const int i = threadIdx.x + r * BS;
       const int pix = i / CH8, ch = (i % CH8) * 8;
       const int yy = ty0 - 1 + pix / (kTileW + 2), xx = tx0 - 1 + pix % (kTileW + 2);
       qin |= (uint32_t)(i < NC && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) << r;
@@ -215,7 +223,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
   if (tile0 < ntiles) load_tile_any(tile0);
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
-    const int i = threadIdx.x + r * kConvThreads;
+    const int i = threadIdx.x + r * BS;
     if (i < NW) *(u32x4*)(sw + (i / (WK / 8)) * SWK + (i % (WK / 8)) * 8) = qw[r];
   }
   for (int64_t tile = tile0; tile < ntiles; tile += gridDim.x) {
@@ -225,25 +233,25 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
     __syncthreads();  // previous tile's output copy done (also orders the weight stores)
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      const int i = threadIdx.x + r * kConvThreads;
+      const int i = threadIdx.x + r * BS;
       if (i < NC) *(u32x4*)(sx + (i / CH8) * S + (i % CH8) * 8) = ((qin >> r) & 1u) ? qs[r] : u32x4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
     if (tile + gridDim.x < ntiles) load_tile_any(tile + gridDim.x);  // in flight during the MFMAs
-    // each wavefront: one tile row, 4 row tiles of 16 pixels, NB column tiles
-    f32x4 acc[4][NB];
+    // each wavefront: one tile row, 4 row tiles of 16 pixels, NBW column tiles
+    f32x4 acc[4][NBW];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int n = 0; n < NB; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int n = 0; n < NBW; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     if constexpr (CI % 32 == 0) {
       // 32-wide K blocks (tap-major, 32 channels each) on the double-K MFMA: lane (lr, lg) holds
       // K elements 8 lg .. 8 lg + 7 of both operands, 16-byte LDS reads
 #pragma unroll
       for (int kb = 0; kb < 9 * CI / 32; ++kb) {
-        bf16x8 bfrag[NB];
+        bf16x8 bfrag[NBW];
 #pragma unroll
-        for (int n = 0; n < NB; ++n) bfrag[n] = *(const bf16x8*)(sw + (n * 16 + lr) * SWK + kb * 32 + 8 * lg);
+        for (int n = 0; n < NBW; ++n) bfrag[n] = *(const bf16x8*)(sw + ((n0 + n) * 16 + lr) * SWK + kb * 32 + 8 * lg);
         const int t = kb / (CI / 32), cb = (kb % (CI / 32)) * 32;
         const int dy = t / 3, dx = t % 3;  // halo offsets 0..2
 #pragma unroll
@@ -251,15 +259,15 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
           const bf16x8 af =
               *(const bf16x8*)(sx + ((wave + dy) * (kTileW + 2) + m * 16 + lr + dx) * S + cb + 8 * lg);
 #pragma unroll
-          for (int n = 0; n < NB; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfrag[n], acc[m][n], 0, 0, 0);
+          for (int n = 0; n < NBW; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfrag[n], acc[m][n], 0, 0, 0);
         }
       }
     } else
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
-      bf16x4 bfrag[NB];
+      bf16x4 bfrag[NBW];
 #pragma unroll
-      for (int n = 0; n < NB; ++n) bfrag[n] = *(const bf16x4*)(sw + (n * 16 + lr) * SWK + kb * 16 + 4 * lg);
+      for (int n = 0; n < NBW; ++n) bfrag[n] = *(const bf16x4*)(sw + ((n0 + n) * 16 + lr) * SWK + kb * 16 + 4 * lg);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         bf16x4 af;
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
           }
         }
 #pragma unroll
-        for (int n = 0; n < NB; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfrag[n], acc[m][n], 0, 0, 0);
+        for (int n = 0; n < NBW; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfrag[n], acc[m][n], 0, 0, 0);
       }
     }
     __syncthreads();  // every wavefront is done with the input tile: reuse it for the output rows
@@ -289,10 +297,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
     {
       const float scale = (OUT_F32 && a.yscale) ? a.yscale[img] : 1.f;
 #pragma unroll
-      for (int n = 0; n < NB; ++n) {
-        const int cl = n * 16 + lr;  // channel inside the block's group
+      for (int n = 0; n < NBW; ++n) {
+        const int cl = (n0 + n) * 16 + lr;  // channel inside the block's group
         if (cl < CB && co0 + cl < CO) {
-          const float bb = bias[n];
+          const float bb = bias[n0 + n];
 #pragma unroll
           for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -317,12 +325,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3(ConvArgs a) {
       if constexpr (OUT_F32) {
         const float* src = (const float*)sbuf + wave * kTileW * CB;
         float* dst = (float*)a.y + row0;
-        for (int i = lane; i < npx * CB; i += 64) dst[(i / CB) * CO + i % CB] = src[i];
+        for (int i = lane + 64 * part; i < npx * CB; i += 64 * WS) dst[(i / CB) * CO + i % CB] = src[i];
       } else {
         constexpr int C8 = CB / 8;  // 16-byte chunks per pixel of this group
         const uint16_t* src = sbuf + wave * kTileW * SOS;
         uint16_t* dst = (uint16_t*)a.y + row0;
-        for (int i = lane; i < npx * C8; i += 64)
+        for (int i = lane + 64 * part; i < npx * C8; i += 64 * WS)
           *(u32x4*)(dst + (i / C8) * CO + (i % C8) * 8) = *(const u32x4*)(src + (i / C8) * SOS + (i % C8) * 8);
       }
     }
