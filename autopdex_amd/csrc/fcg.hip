@@ -102,9 +102,13 @@ __global__ __launch_bounds__(kFinBS) void k_fcg_beta_fin(KState* S, const double
   if (threadIdx.x == 0) S->beta = S->k > 0 ? -s[0] / S->gamma_prev : 0.0;
 }
 
-template <class T>
+// columns: int32 as stored, or int16 offsets from the lane's row (the last row for lanes past n)
+__device__ __forceinline__ int32_t fcg_col(int32_t c, int32_t) { return c; }
+__device__ __forceinline__ int32_t fcg_col(int16_t d, int32_t base) { return base + (int32_t)d; }
+
+template <class T, class C>
 __global__ __launch_bounds__(kSpmvBS) void k_fcg_spmv(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
-                                                      const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                      const C* __restrict__ cols, const T* __restrict__ vals,
                                                       const double* __restrict__ z, double* __restrict__ pa,
                                                       double* __restrict__ pb, double* __restrict__ q,
                                                       const double* __restrict__ r, KState* S, double* partials) {
@@ -118,15 +122,19 @@ __global__ __launch_bounds__(kSpmvBS) void k_fcg_spmv(int64_t n, int64_t n_chunk
   for (int64_t c = blockIdx.x * (int64_t)(kSpmvBS / 64) + (threadIdx.x >> 6); c < n_chunks;
        c += (int64_t)gridDim.x * (kSpmvBS / 64)) {
     const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
+    const int64_t row = c * 64 + lane;
+    const int32_t base = (int32_t)(row < n ? row : n - 1);
     const T* vv = vals + off + lane;
-    const int32_t* ci = cols + off + lane;
+    const C* ci = cols + off + lane;
     // values and columns are read once: non-temporal; four entries' loads are issued before
     // their gathers (more loads in flight), the sum keeps the entry order
     double s = 0.0;
     int k = 0;
     for (; k + 4 <= w; k += 4) {
-      const int32_t c0 = __builtin_nontemporal_load(ci + (k + 0) * 64), c1 = __builtin_nontemporal_load(ci + (k + 1) * 64),
-                    c2 = __builtin_nontemporal_load(ci + (k + 2) * 64), c3 = __builtin_nontemporal_load(ci + (k + 3) * 64);
+      const int32_t c0 = fcg_col(__builtin_nontemporal_load(ci + (k + 0) * 64), base),
+                    c1 = fcg_col(__builtin_nontemporal_load(ci + (k + 1) * 64), base),
+                    c2 = fcg_col(__builtin_nontemporal_load(ci + (k + 2) * 64), base),
+                    c3 = fcg_col(__builtin_nontemporal_load(ci + (k + 3) * 64), base);
       const T v0 = __builtin_nontemporal_load(vv + (k + 0) * 64), v1 = __builtin_nontemporal_load(vv + (k + 1) * 64),
               v2 = __builtin_nontemporal_load(vv + (k + 2) * 64), v3 = __builtin_nontemporal_load(vv + (k + 3) * 64);
       s += (double)v0 * (z[c0] + beta * po[c0]);
@@ -135,10 +143,9 @@ __global__ __launch_bounds__(kSpmvBS) void k_fcg_spmv(int64_t n, int64_t n_chunk
       s += (double)v3 * (z[c3] + beta * po[c3]);
     }
     for (; k < w; ++k) {
-      const int32_t col = __builtin_nontemporal_load(ci + k * 64);
+      const int32_t col = fcg_col(__builtin_nontemporal_load(ci + k * 64), base);
       s += (double)__builtin_nontemporal_load(vv + k * 64) * (z[col] + beta * po[col]);
     }
-    const int64_t row = c * 64 + lane;
     if (row < n) {
       const double pi = z[row] + beta * po[row];
       pn[row] = pi;
@@ -238,8 +245,12 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
     // timed: the pair takes the SpMV's own start and end timestamps
     const EventPair e1 = P.timing ? timing_pair(P) : EventPair{};
     t_launch_events = P.timing ? &e1 : nullptr;
-    launch_kernel(k_fcg_spmv<T>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p, vals, z,
-                  pa, pb, q, r, S, P.partials.p);
+    if (P.sell_dcol.p)
+      launch_kernel(k_fcg_spmv<T, int16_t>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p,
+                    (const int16_t*)P.sell_dcol.p, vals, z, pa, pb, q, r, S, P.partials.p);
+    else
+      launch_kernel(k_fcg_spmv<T, int32_t>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p,
+                    (const int32_t*)P.sell_cols.p, vals, z, pa, pb, q, r, S, P.partials.p);
     hipLaunchKernelGGL(k_fcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gs);
     hipLaunchKernelGGL(k_fcg_update, dim3(gv), dim3(kBS), 0, st, n, pa, pb, q, x, r, S, P.partials.p);
     hipLaunchKernelGGL(k_fcg_update_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv);
@@ -308,7 +319,7 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   if (hs.reliable == 2) res.stagnated = true;  // breakdown
   if (P.timing) {
     const size_t k = (size_t)hs.k;
-    const int64_t spmv_bytes = P.sell_total * (int64_t)(sizeof(T) + 4) + (P.n_chunks + 1) * 4 + n * 6 * 8;
+    const int64_t spmv_bytes = P.sell_total * (int64_t)(sizeof(T) + (P.sell_dcol.p ? 2 : 4)) + (P.n_chunks + 1) * 4 + n * 6 * 8;
     timing_collect(P, "fcg_precond", ev_net, 0, k);
     timing_collect(P, sizeof(T) == 8 ? "fcg_spmv_f64" : "fcg_spmv_f32", ev_spmv, spmv_bytes, k);
     N.collect_timing(k);  // the forward passes of the executed iterations

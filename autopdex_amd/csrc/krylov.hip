@@ -28,6 +28,7 @@
 // cover HBM latency; grids are sized to about one pass per thread.
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 
 #include "assembly.hpp"
 
@@ -103,53 +104,68 @@ __device__ inline void finalize_sums(const double* partials, int nb, double (&su
   }
 }
 
-// Row gather of one SELL chunk: sum_k vals[k] * f(cols[k]) for this lane's row,
+// Column of a stored entry: an int32 column as stored, or an int16 offset from the row (base)
+__device__ __forceinline__ int32_t sell_col(int32_t c, int32_t) { return c; }
+__device__ __forceinline__ int32_t sell_col(int16_t d, int32_t base) { return base + (int32_t)d; }
+// The row an int16 offset is taken from: the lane's row, or the last row for lanes past n
+__device__ __forceinline__ int32_t sell_base(int64_t row, int64_t n) { return (int32_t)(row < n ? row : n - 1); }
+
+// Row gather of one SELL chunk: sum_k vals[k] * f(col[k]) for this lane's row,
 // values of type M, accumulated in V (FP32 values with FP64 vectors: each
 // value is widened exactly and the products and sums are FP64).
 // U entries per step: all their column and value loads are issued before the gathers (U = 8
 // for the wide rows of 3-DOF elasticity, 4 otherwise); the sum order is k = 0, 1, 2, ... either way.
-template <class M, class V = M, int U = 4, class F>
-__device__ __forceinline__ V sell_row(const M* __restrict__ v, const int32_t* __restrict__ ci, int w, F f) {
+template <class M, class V = M, int U = 4, class C, class F>
+__device__ __forceinline__ V sell_row(const M* __restrict__ v, const C* __restrict__ ci, int w, int32_t base, F f) {
   V s = 0;
   int k = 0;
   for (; k + U <= w; k += U) {
     // values and columns are read once per SpMV: non-temporal, so L2 keeps the gathered vector
-    int32_t c[U];
+    C c[U];
     M vv[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) c[j] = __builtin_nontemporal_load(ci + (k + j) * 64);
 #pragma unroll
     for (int j = 0; j < U; ++j) vv[j] = __builtin_nontemporal_load(v + (k + j) * 64);
 #pragma unroll
-    for (int j = 0; j < U; ++j) s += (V)vv[j] * f(c[j]);
+    for (int j = 0; j < U; ++j) s += (V)vv[j] * f(sell_col(c[j], base));
   }
-  for (; k < w; ++k) s += (V)__builtin_nontemporal_load(v + k * 64) * f(__builtin_nontemporal_load(ci + k * 64));
+  for (; k < w; ++k)
+    s += (V)__builtin_nontemporal_load(v + k * 64) * f(sell_col(__builtin_nontemporal_load(ci + k * 64), base));
   return s;
 }
 
 // ---- SpMV (SELL-64: one wavefront per chunk, lane = row) --------------------------
-template <class T>
+template <class T, class C>
 __global__ __launch_bounds__(kSpmvBS) void k_spmv_sell(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
-                                                       const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                       const C* __restrict__ cols, const T* __restrict__ vals,
                                                        const T* __restrict__ x, T* __restrict__ y) {
   constexpr int wpb = kSpmvBS / 64;
   const int lane = threadIdx.x & 63;
   for (int64_t c = blockIdx.x * (int64_t)wpb + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * wpb) {
     const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
-    const T s = sell_row<T>(vals + off + lane, cols + off + lane, w, [&](int32_t col) { return x[col]; });
     const int64_t row = c * 64 + lane;
+    const T s = sell_row<T>(vals + off + lane, cols + off + lane, w, sell_base(row, n),
+                            [&](int32_t col) { return x[col]; });
     if (row < n) y[row] = s;
   }
 }
 
 void spmv(Plan& P, const void* vals, const void* x, void* y, int precision, hipStream_t st) {
   const int g = grid_chunks(P.n_chunks);
-  if (precision == APX_FP64)
-    hipLaunchKernelGGL(k_spmv_sell<double>, dim3(g), dim3(kSpmvBS), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p,
-                       P.sell_cols.p, (const double*)vals, (const double*)x, (double*)y);
+  auto run = [&](auto cols) {
+    using C = std::remove_const_t<std::remove_pointer_t<decltype(cols)>>;
+    if (precision == APX_FP64)
+      hipLaunchKernelGGL((k_spmv_sell<double, C>), dim3(g), dim3(kSpmvBS), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p,
+                         cols, (const double*)vals, (const double*)x, (double*)y);
+    else
+      hipLaunchKernelGGL((k_spmv_sell<float, C>), dim3(g), dim3(kSpmvBS), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p,
+                         cols, (const float*)vals, (const float*)x, (float*)y);
+  };
+  if (P.sell_dcol.p)
+    run((const int16_t*)P.sell_dcol.p);
   else
-    hipLaunchKernelGGL(k_spmv_sell<float>, dim3(g), dim3(kSpmvBS), 0, st, P.n_free, P.n_chunks, P.sell_ptr.p,
-                       P.sell_cols.p, (const float*)vals, (const float*)x, (float*)y);
+    run((const int32_t*)P.sell_cols.p);
   APX_LAUNCH_CHECK();
 }
 
@@ -271,9 +287,9 @@ __global__ __launch_bounds__(kFinBS) void k_pcg_init_fin(KState* S, const double
   }
 }
 
-template <class M, class T, int U = 4>
+template <class M, class T, int U, class C>
 __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunks, const int32_t* __restrict__ sptr,
-                                                      const int32_t* __restrict__ cols, const M* __restrict__ vals,
+                                                      const C* __restrict__ cols, const M* __restrict__ vals,
                                                       const T* __restrict__ z, T* __restrict__ pa, T* __restrict__ pb,
                                                       T* __restrict__ q, KState* S, double* partials) {
   if (halted(S)) return;
@@ -286,9 +302,9 @@ __global__ __launch_bounds__(kSpmvBS) void k_pcg_spmv(int64_t n, int64_t n_chunk
   double v[1] = {0.0};
   for (int64_t c = blockIdx.x * (int64_t)wpb + (threadIdx.x >> 6); c < n_chunks; c += (int64_t)gridDim.x * wpb) {
     const int32_t off = sptr[c], w = (sptr[c + 1] - off) >> 6;
-    const T s = sell_row<M, T, U>(vals + off + lane, cols + off + lane, w,
-                                  [&](int32_t col) { return z[col] + beta * po[col]; });
     const int64_t row = c * 64 + lane;
+    const T s = sell_row<M, T, U>(vals + off + lane, cols + off + lane, w, sell_base(row, n),
+                                  [&](int32_t col) { return z[col] + beta * po[col]; });
     if (row < n) {
       const T pi = z[row] + beta * po[row];
       pn[row] = pi;
@@ -405,8 +421,9 @@ PcgRun<M, T>::PcgRun(Plan& P_, const M* vals_, const T* inv_, hipStream_t st_)
   part = P.partials.p;
   APX_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   APX_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
-  // SELL values + columns, chunk offsets, z and p_old read, p_new and q written
-  spmv_bytes = P.sell_total * (int64_t)(sizeof(M) + 4) + (P.n_chunks + 1) * 4 + n * 4 * (int64_t)sizeof(T);
+  // SELL values + columns (int16 offsets or int32), chunk offsets, z and p_old read, p_new and q written
+  spmv_bytes = P.sell_total * (int64_t)(sizeof(M) + (P.sell_dcol.p ? 2 : 4)) + (P.n_chunks + 1) * 4 +
+               n * 4 * (int64_t)sizeof(T);
   upd_bytes = n * 8 * (int64_t)sizeof(T);  // reads D^-1, p, q, x, r; writes x, r, z
 }
 
@@ -441,12 +458,19 @@ void PcgRun<M, T>::launch_iteration() {
   // timed: the pairs take the SpMV's and the update's own start and end timestamps
   const EventPair e1 = timed ? timing_pair(P) : EventPair{}, e2 = timed ? timing_pair(P) : EventPair{};
   t_launch_events = timed ? &e1 : nullptr;
-  if (P.sell_total >= 32 * (int64_t)P.n_free)  // rows of 32+ stored entries: deeper load batches
-    launch_kernel((k_pcg_spmv<M, T, 8>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
-                  vals, z, pa, pb, q, S, part);
+  auto spmv = [&](auto cols) {
+    using C = std::remove_const_t<std::remove_pointer_t<decltype(cols)>>;
+    if (P.sell_total >= 32 * (int64_t)P.n_free)  // rows of 32+ stored entries: deeper load batches
+      launch_kernel((k_pcg_spmv<M, T, 8, C>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, cols, vals,
+                    z, pa, pb, q, S, part);
+    else
+      launch_kernel((k_pcg_spmv<M, T, 4, C>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, cols, vals,
+                    z, pa, pb, q, S, part);
+  };
+  if (P.sell_dcol.p)
+    spmv((const int16_t*)P.sell_dcol.p);
   else
-    launch_kernel((k_pcg_spmv<M, T, 4>), dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p, P.sell_cols.p,
-                  vals, z, pa, pb, q, S, part);
+    spmv((const int32_t*)P.sell_cols.p);
   hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, part, gs);
   t_launch_events = timed ? &e2 : nullptr;
   launch_kernel(k_pcg_update<T>, dim3(gu), dim3(kBS), 0, st, n, inv, pa, pb, q, x, r, z, S, part);

@@ -75,9 +75,9 @@ void timing_collect(Plan& P, const char* name, std::vector<EventPair>& pairs, in
 
 int64_t Plan::device_bytes() const {
   int64_t b = coords.bytes() + dmask.bytes() + full2red.bytes() + red2full.bytes() + indptr.bytes() +
-              indices.bytes() + diag_slot.bytes() + sell_ptr.bytes() + sell_cols.bytes() + csr2sell.bytes() +
-              vals64.bytes() + vals32.bytes() + w64.bytes() +
-              w32.bytes() + r_full.bytes() + diag_full.bytes() + partials.bytes();
+              indices.bytes() + diag_slot.bytes() + sell_ptr.bytes() + sell_cols.bytes() + sell_dcol.bytes() +
+              csr2sell.bytes() + vals64.bytes() + vals32.bytes() + w64.bytes() + w32.bytes() + r_full.bytes() +
+              diag_full.bytes() + partials.bytes();
   for (auto& s : sets) b += s.conn.bytes() + s.slots.bytes();
   return b;
 }
@@ -302,7 +302,30 @@ void plan_finalize(Plan& P) {
   }
   P.diag_slot.upload(ds.data(), ds.size());
   P.sell_ptr.upload(P.sell_ptr_h.data(), P.sell_ptr_h.size());
-  P.sell_cols.upload(P.sell_cols_h.data(), P.sell_cols_h.size());
+  {
+    // int16 column offsets when every stored entry's offset from its row fits
+    std::vector<int16_t> dcol(P.sell_total, 0);
+    bool fits = true;
+    for (int64_t r = 0; r < P.n_free && fits; ++r) {
+      const int64_t c = r / kSellC, lane = r % kSellC;
+      for (int32_t k = P.indptr_h[r]; k < P.indptr_h[r + 1]; ++k) {
+        const int64_t s = P.sell_ptr_h[c] + (int64_t)(k - P.indptr_h[r]) * kSellC + lane;
+        const int64_t d = (int64_t)P.sell_cols_h[s] - r;
+        if (d < INT16_MIN || d > INT16_MAX) {
+          fits = false;
+          break;
+        }
+        dcol[s] = (int16_t)d;
+      }
+    }
+    if (fits) {
+      P.sell_dcol.upload(dcol.data(), dcol.size());
+      P.sell_cols = DevBuf<int32_t>();
+    } else {
+      P.sell_dcol = DevBuf<int16_t>();
+      P.sell_cols.upload(P.sell_cols_h.data(), P.sell_cols_h.size());
+    }
+  }
   P.csr2sell.upload(P.csr2sell_h.data(), P.csr2sell_h.size());
   std::vector<int32_t>().swap(P.sell_cols_h);
   std::vector<int32_t>().swap(P.csr2sell_h);

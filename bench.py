@@ -68,8 +68,9 @@ HBM_PEAK = 8.0e12
 # reading of a 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000).
 # The convolution's figure is the average over the launches of one network application (two at
 # 513 x 513, four at 257 x 257), the mix its timer averages over.
-PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double, double, 4>", 2048): 660.2e6, ("k_pcg_spmv<float, double, 4>", 2048): 509.5e6,
-                     ("k_pcg_spmv<float, float, 4>", 2048): 408.5e6, ("k_fcg_spmv<float>", 2048): 543.3e6,
+PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double, double, 4, short>", 2048): 584.8e6,
+                     ("k_pcg_spmv<float, double, 4, short>", 2048): 434.1e6,
+                     ("k_pcg_spmv<float, float, 4, short>", 2048): 333.1e6, ("k_fcg_spmv<float, short>", 2048): 467.7e6,
                      ("k_conv3x3<64, 64, false, 4, 2>", 2048): 37.33e6, ("k_conv3x3_ws<16, 16, 4, 0>", 2048): 292.1e6}
 
 
@@ -154,9 +155,11 @@ def all_timers(prob):
 
 
 # single-kernel solver timers and the kernel each one times (rocprofv3 names)
-SOLVER_KERNELS = {"fcg_spmv_f32": "k_fcg_spmv<float>", "fcg_spmv_f64": "k_fcg_spmv<double>",
-                  "pcg_spmv_f64": "k_pcg_spmv<double, double, 4>", "pcg_spmv_f32": "k_pcg_spmv<float, float, 4>",
-                  "pcg_spmv_f32m_f64v": "k_pcg_spmv<float, double, 4>", "pcg_update_f64": "k_pcg_update<double>",
+# (the SpMVs of the 2048^2 grid read int16 column offsets: template argument "short")
+SOLVER_KERNELS = {"fcg_spmv_f32": "k_fcg_spmv<float, short>", "fcg_spmv_f64": "k_fcg_spmv<double, short>",
+                  "pcg_spmv_f64": "k_pcg_spmv<double, double, 4, short>",
+                  "pcg_spmv_f32": "k_pcg_spmv<float, float, 4, short>",
+                  "pcg_spmv_f32m_f64v": "k_pcg_spmv<float, double, 4, short>", "pcg_update_f64": "k_pcg_update<double>",
                   "pcg_update_f32": "k_pcg_update<float>"}
 
 

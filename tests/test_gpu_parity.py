@@ -108,15 +108,26 @@ def test_assembly_is_bitwise_deterministic(torch_gpu):
     assert outs[0] == outs[1] == outs[2]
 
 
-def test_spmv_and_pcg_match_oracle(torch_gpu):
+# The 40 x 40 grid's SELL columns are stored as int16 offsets from their rows; the rows of the
+# 3 x 40000 strip (unit-square elements) reach 40,000 columns away, past int16, so it takes the
+# int32 columns.
+STRIP = ((0.0, 0.0), (3.0, 0.0), (3.0, 40000.0), (0.0, 40000.0))
+
+
+@pytest.mark.parametrize("builder,args,int16_cols",
+                         [(P.poisson_source, (40,), True), (P.poisson_rect, (3, 40000, STRIP), False)],
+                         ids=["grid40_int16_columns", "strip3x40000_int32_columns"])
+def test_spmv_and_pcg_match_oracle(torch_gpu, builder, args, int16_cols):
     from autopdex_amd import _native as N
     from autopdex_amd.backend import current_stream
     from oracle import assembly as OA
     from oracle import newton as ON
     torch = torch_gpu
-    dofs, settings, static, o, prob = _setup(P.poisson_source, 40)
+    dofs, settings, static, o, prob = _setup(builder, *args)
     u = np.zeros(prob.n_dofs)
     A = OA.reduced_tangent(o["sets"], o["coords"], u, 1, ~o["mask"])
+    rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+    assert (np.abs(A.indices - rows).max() <= 32767) == int16_cols
     st = current_stream()
     vals = torch.from_numpy(A.data.copy()).cuda()
     x = np.random.default_rng(3).standard_normal(prob.n_free)
