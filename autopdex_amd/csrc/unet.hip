@@ -46,24 +46,33 @@ int64_t resident_blocks(K kernel, int threads = kConvThreads) {
   return (int64_t)std::max(1, cus) * std::max(1, per_cu);
 }
 
-template <int CI, int CO, bool F32>
+template <int CI, int CO, bool F32, bool XM>
 void launch_conv_t(const ConvArgs& a, hipStream_t st) {
   constexpr int NT = round16(CO) / 16;
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
   if (NT > 1 && tiles < kSplitTiles) {
-    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, 1>);
+    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, 1, 1, XM>);
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap / NT));
-    launch_kernel((k_conv3x3<CI, CO, F32, 1>), dim3(g, NT), dim3(kConvThreads), 0, st, a);
+    launch_kernel((k_conv3x3<CI, CO, F32, 1, 1, XM>), dim3(g, NT), dim3(kConvThreads), 0, st, a);
   } else if constexpr (NT == 4) {
     // four channel tiles: two waves per tile row (WS = 2), two tiles each
-    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT, 2>, 2 * kConvThreads);
+    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT, 2, XM>, 2 * kConvThreads);
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
-    launch_kernel((k_conv3x3<CI, CO, F32, NT, 2>), dim3(g), dim3(2 * kConvThreads), 0, st, a);
+    launch_kernel((k_conv3x3<CI, CO, F32, NT, 2, XM>), dim3(g), dim3(2 * kConvThreads), 0, st, a);
   } else {
-    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT>);
+    static const int64_t cap = resident_blocks(k_conv3x3<CI, CO, F32, NT, 1, XM>);
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
-    launch_kernel((k_conv3x3<CI, CO, F32, NT>), dim3(g), dim3(kConvThreads), 0, st, a);
+    launch_kernel((k_conv3x3<CI, CO, F32, NT, 1, XM>), dim3(g), dim3(kConvThreads), 0, st, a);
   }
+}
+
+// the ReLU-masked input (backward data pass) and the plain one are separate kernels
+template <int CI, int CO, bool F32>
+void launch_conv_t(const ConvArgs& a, hipStream_t st) {
+  if (a.xmask)
+    launch_conv_t<CI, CO, F32, true>(a, st);
+  else
+    launch_conv_t<CI, CO, F32, false>(a, st);
 }
 
 template <int CI, int CO, int TR, int IN = 0>
@@ -151,12 +160,13 @@ void launch_wgrad(int ci, int co, const WgradArgs& a, hipStream_t st) {
 std::string conv_kernel_name(int ci, int co, bool f32, const ConvArgs& a) {
   auto s = [](int v) { return std::to_string(v); };
   if (f32) return "k_conv_out<" + s(co) + ">";
+  const std::string xm = a.xmask ? "true>" : "false>";
   if ((ci == 16 || ci == 32) && (co == 16 || co == 32)) return "k_conv3x3_ws<" + s(ci) + ", " + s(co) + ", 4, 0>";
   if ((ci == 1 || ci == 2) && co == 16 && !a.xmask) return "k_conv_in<" + s(ci) + ">";
   const int nt = round16(co) / 16;
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
   const int nb = (nt > 1 && tiles < kSplitTiles) ? 1 : nt;
-  return "k_conv3x3<" + s(ci) + ", " + s(co) + ", false, " + s(nb) + (nb == 4 ? ", 2>" : ", 1>");
+  return "k_conv3x3<" + s(ci) + ", " + s(co) + ", false, " + s(nb) + (nb == 4 ? ", 2, " : ", 1, ") + xm;
 }
 
 // ---- grid transfer ---------------------------------------------------------------------
@@ -354,12 +364,6 @@ constexpr int kTailPix = (kTailMaxGrid + 2) * (kTailMaxGrid + 2);
 constexpr int kTailStage = (kTailPix * 8 + kTailThreads - 1) / kTailThreads;  // staged input chunks per thread
 constexpr int kTailWChunks = 64 * 576 / 8 / kTailThreads;                      // weight chunks per thread
 static_assert(64 * 576 / 8 % kTailThreads == 0, "weight chunks must tile the workgroup");
-
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 template <int NM>
 __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint16_t* sw, int m0, int H, int W, int cp,

@@ -44,6 +44,16 @@ __device__ __forceinline__ uint32_t f2bf2(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2_t));
 }
 
+// Block barrier that orders LDS accesses only.  __syncthreads() also waits for every
+// outstanding global load and store of the wave (vmcnt 0), which puts the previous tile's
+// output stores and any prefetch in series with the barrier.  The waves of k_conv3x3's tile
+// loop and of the coarse-level launch share only LDS, so their barriers need only this one.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kTileW = 64;  // output pixels per tile row (4 MFMA row tiles)
 constexpr int kTileH = 4;   // tile rows = wavefronts per block
 constexpr int kConvThreads = 64 * kTileH;
@@ -75,9 +85,11 @@ struct ConvArgs {
 // 16 channels; CI < 16 -> k = t*CI + c over ceil(9 CI / 16) blocks.
 //
 // Memory path: the input tile (one-pixel halo) and the weights are staged in
-// LDS with 16-byte global loads and 16-byte LDS stores.  The accumulators go
-// back through LDS so that every wavefront stores its output row -- 64 pixels x
-// CO channels, contiguous in NHWC -- with 16-byte stores.
+// LDS with 16-byte global loads and 16-byte LDS stores.  bf16 outputs leave the
+// accumulators directly: with the weights as the MFMA's A operand a lane holds four
+// consecutive channels of one pixel, one 8-byte store (four lanes cover 32 contiguous
+// bytes of a pixel), so a tile needs no output pass through LDS and no barriers
+// around one.
 //
 // LDS strides.  A 16-byte LDS read (the fragments of the 16x16x32 MFMA) is served
 // for 32 lanes at a time over 128 four-byte banks (measured on gfx950 with
@@ -108,8 +120,11 @@ __host__ __device__ constexpr int conv_lds_stride() {
 // threads); with WS = 2 each row's NB channel tiles are split between two waves,
 // so that two waves share each SIMD and one issues MFMAs while the other waits
 // on its LDS fragments (the block's LDS, which allows one block per compute unit,
-// is the same).
-template <int CI, int CO, bool OUT_F32, int NB, int WS = 1>
+// is the same).  XM: the input is multiplied by (xmask > 0) (the ReLU backward of the data
+// pass); a template parameter, so that each kernel has one load path (loads on two paths of a
+// runtime branch made the compiler wait for all of them where the paths join, before the MFMAs
+// they were meant to overlap).
+template <int CI, int CO, bool OUT_F32, int NB, int WS = 1, bool XM = false>
 __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
   constexpr int BS = kConvThreads * WS;
   constexpr int NBW = NB / WS;  // channel tiles per wavefront
@@ -123,10 +138,14 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
   static_assert(NT % NB == 0, "channel groups must tile the output channels");
   constexpr int CB = (NB == NT) ? CO : 16 * NB;  // output channels of this block
   constexpr int SX = (kTileH + 2) * (kTileW + 2) * S;
-  // output staging: pixel stride CB + 8 bf16 (16 bytes of padding), so that the four lane groups
-  // of a fragment store, four pixels apart, land on different LDS banks
-  constexpr int SOS = OUT_F32 ? CB : CB + 8;
-  constexpr int SO = kTileH * kTileW * SOS * (OUT_F32 ? 2 : 1);  // output staging (uint16 units)
+  // bf16 output: the MFMAs form D^T = W X^T (the weights as the A operand), so that a lane holds
+  // output channels 4 lg .. 4 lg + 3 of pixel lr and stores them with one 8-byte write, straight
+  // from the accumulators.  FP32 output: D = X W^T (a lane holds one channel of pixels
+  // 4 lg .. 4 lg + 3), written through LDS rows.  The products and their order are the same.
+  constexpr bool kDirect = !OUT_F32;
+  static_assert(!kDirect || CO % 4 == 0, "direct stores write four channels at a time");
+  // output staging (FP32 output only): pixel stride CB
+  constexpr int SO = kDirect ? 0 : kTileH * kTileW * CB * 2;  // uint16 units
   constexpr int SBUF = SX > SO ? SX : SO;
   constexpr int CH8 = (CI + 7) / 8;  // 16-byte input chunks per pixel
   constexpr int NP = (kTileH + 2) * (kTileW + 2);
@@ -147,12 +166,18 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
   // the biases of this lane's output channels, once per block and ahead of the weights, so that
   // the wait for the weights covers them (a load used in the epilogue would otherwise be waited
   // for with every tile, behind the next tile's prefetch)
-  float bias[NB];
+  const int wave = (threadIdx.x >> 6) % kTileH, part = (threadIdx.x >> 6) / kTileH, lane = threadIdx.x & 63;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int n0 = part * NBW;  // this wave's first channel tile
+  constexpr int NBIAS = kDirect ? 4 : 1;  // output channels per lane and channel tile
+  float bias[NBW][NBIAS];
 #pragma unroll
-  for (int n = 0; n < NB; ++n) {
-    const int cl = n * 16 + (int)(threadIdx.x & 15);
-    bias[n] = (a.bias && cl < CB && co0 + cl < CO) ? a.bias[co0 + cl] : 0.f;
-  }
+  for (int n = 0; n < NBW; ++n)
+#pragma unroll
+    for (int j = 0; j < NBIAS; ++j) {
+      const int cl = (n0 + n) * 16 + (kDirect ? 4 * lg + j : lr);
+      bias[n][j] = (a.bias && cl < CB && co0 + cl < CO) ? a.bias[co0 + cl] : 0.f;
+    }
   u32x4 qw[RW];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
@@ -161,9 +186,6 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
   }
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + kTileH - 1) / kTileH;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
-  const int wave = (threadIdx.x >> 6) % kTileH, part = (threadIdx.x >> 6) / kTileH, lane = threadIdx.x & 63;
-  const int lr = lane & 15, lg = lane >> 4;
-  const int n0 = part * NBW;  // this wave's first channel tile
   // input tile loads with a one-pixel halo into registers; bit r of qin: chunk r lies inside
   // the image (otherwise it is stored as zeros)
   static_assert(RC <= 32, "one validity bit per staged chunk");
@@ -214,10 +236,7 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
     }
   };
   auto load_tile_any = [&](int64_t tile) __attribute__((always_inline)) {
-    if (a.xmask)
-      load_tile(tile, std::true_type{});
-    else
-      load_tile(tile, std::false_type{});
+    load_tile(tile, std::integral_constant<bool, XM>{});
   };
   const int64_t tile0 = xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
   if (tile0 < ntiles) load_tile_any(tile0);
@@ -230,13 +249,26 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
     const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;
-    __syncthreads();  // previous tile's output copy done (also orders the weight stores)
+    // every wave is done with the previous tile (also orders the weight stores); the direct
+    // output stores of that tile stay in flight
+    if constexpr (kDirect)
+      lds_barrier();
+    else
+      __syncthreads();
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
       const int i = threadIdx.x + r * BS;
       if (i < NC) *(u32x4*)(sx + (i / CH8) * S + (i % CH8) * 8) = ((qin >> r) & 1u) ? qs[r] : u32x4{0u, 0u, 0u, 0u};
     }
-    __syncthreads();
+    // every global access issued so far has completed (the staging needed the loads; the previous
+    // tile's stores are older still): an unconditional wait here, because the compiler's wait for
+    // the last chunk sits under its per-lane branch, and otherwise the first fragment reads below
+    // (into registers that the previous tile's stores read from) wait for the prefetch too
+    __builtin_amdgcn_s_waitcnt(0);
+    if constexpr (kDirect)
+      lds_barrier();
+    else
+      __syncthreads();
     if (tile + gridDim.x < ntiles) load_tile_any(tile + gridDim.x);  // in flight during the MFMAs
     // each wavefront: one tile row, 4 row tiles of 16 pixels, NBW column tiles
     f32x4 acc[4][NBW];
@@ -259,7 +291,9 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
           const bf16x8 af =
               *(const bf16x8*)(sx + ((wave + dy) * (kTileW + 2) + m * 16 + lr + dx) * S + cb + 8 * lg);
 #pragma unroll
-          for (int n = 0; n < NBW; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfrag[n], acc[m][n], 0, 0, 0);
+          for (int n = 0; n < NBW; ++n)
+            acc[m][n] = kDirect ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfrag[n], af, acc[m][n], 0, 0, 0)
+                                : __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfrag[n], acc[m][n], 0, 0, 0);
         }
       }
     } else
@@ -289,49 +323,64 @@ __global__ __launch_bounds__(kConvThreads * WS) void k_conv3x3(ConvArgs a) {
           }
         }
 #pragma unroll
-        for (int n = 0; n < NBW; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfrag[n], acc[m][n], 0, 0, 0);
+        for (int n = 0; n < NBW; ++n)
+          acc[m][n] = kDirect ? __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bfrag[n], af, acc[m][n], 0, 0, 0)
+                              : __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfrag[n], acc[m][n], 0, 0, 0);
       }
     }
-    __syncthreads();  // every wavefront is done with the input tile: reuse it for the output rows
-    // epilogue: bias, activation -> LDS row of this wavefront (lane holds co = co0+n*16+lr, pixels 4*lg+j)
-    {
-      const float scale = (OUT_F32 && a.yscale) ? a.yscale[img] : 1.f;
+    const int yy = ty0 + wave;
+    if constexpr (kDirect) {
+      // bias, activation, bf16, one 8-byte store per (row tile, channel tile): lane (lr, lg) holds
+      // channels 4 lg .. 4 lg + 3 of pixel m * 16 + lr
+      if (yy < a.H) {
+        uint16_t* yrow = (uint16_t*)a.y + (((int64_t)img * a.H + yy) * a.W + tx0) * CO + co0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int px = m * 16 + lr;
+          if (tx0 + px < a.W) {
+#pragma unroll
+            for (int n = 0; n < NBW; ++n) {
+              const int cl = (n0 + n) * 16 + 4 * lg;  // first of this lane's four channels
+              if (cl < CB && co0 + cl < CO) {
+                float v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  v[j] = acc[m][n][j] + bias[n][j];
+                  if (a.relu) v[j] = fmaxf(v[j], 0.f);
+                }
+                *(uint2*)(yrow + (int64_t)px * CO + cl) = uint2{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])};
+              }
+            }
+          }
+        }
+      }
+    } else {
+      __syncthreads();  // every wavefront is done with the input tile: reuse it for the output rows
+      // epilogue: bias, activation, scale -> LDS row of this wavefront (lane holds co = co0+n*16+lr,
+      // pixels 4*lg+j)
+      const float scale = a.yscale ? a.yscale[img] : 1.f;
 #pragma unroll
       for (int n = 0; n < NBW; ++n) {
         const int cl = (n0 + n) * 16 + lr;  // channel inside the block's group
         if (cl < CB && co0 + cl < CO) {
-          const float bb = bias[n0 + n];
 #pragma unroll
           for (int m = 0; m < 4; ++m)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              float v = acc[m][n][j] + bb;
+              float v = acc[m][n][j] + bias[n][0];
               if (a.relu) v = fmaxf(v, 0.f);
-              const int px = m * 16 + 4 * lg + j;
-              if constexpr (OUT_F32)
-                ((float*)sbuf)[(wave * kTileW + px) * CB + cl] = v * scale;
-              else
-                sbuf[(wave * kTileW + px) * SOS + cl] = f2bf(v);
+              ((float*)sbuf)[(wave * kTileW + m * 16 + 4 * lg + j) * CB + cl] = v * scale;
             }
         }
       }
-    }
-    __syncthreads();
-    // copy out: each wavefront's row holds 64 pixels x CB channels (contiguous in NHWC when CB = CO)
-    const int yy = ty0 + wave;
-    const int npx = min(kTileW, a.W - tx0);
-    if (yy < a.H) {
-      const int64_t row0 = (((int64_t)img * a.H + yy) * a.W + tx0) * CO + co0;
-      if constexpr (OUT_F32) {
+      __syncthreads();
+      // copy out: each wavefront's row holds 64 pixels x CB channels (contiguous in NHWC when CB = CO)
+      const int npx = min(kTileW, a.W - tx0);
+      if (yy < a.H) {
+        const int64_t row0 = (((int64_t)img * a.H + yy) * a.W + tx0) * CO + co0;
         const float* src = (const float*)sbuf + wave * kTileW * CB;
         float* dst = (float*)a.y + row0;
         for (int i = lane + 64 * part; i < npx * CB; i += 64 * WS) dst[(i / CB) * CO + i % CB] = src[i];
-      } else {
-        constexpr int C8 = CB / 8;  // 16-byte chunks per pixel of this group
-        const uint16_t* src = sbuf + wave * kTileW * SOS;
-        uint16_t* dst = (uint16_t*)a.y + row0;
-        for (int i = lane + 64 * part; i < npx * C8; i += 64 * WS)
-          *(u32x4*)(dst + (i / C8) * CO + (i % C8) * 8) = *(const u32x4*)(src + (i / C8) * SOS + (i % C8) * 8);
       }
     }
   }

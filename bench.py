@@ -71,7 +71,8 @@ HBM_PEAK = 8.0e12
 PMC_TRAFFIC_BYTES = {("k_pcg_spmv<double, double, 4, short>", 2048): 584.8e6,
                      ("k_pcg_spmv<float, double, 4, short>", 2048): 434.1e6,
                      ("k_pcg_spmv<float, float, 4, short>", 2048): 333.1e6, ("k_fcg_spmv<float, short>", 2048): 467.7e6,
-                     ("k_conv3x3<64, 64, false, 4, 2>", 2048): 37.33e6, ("k_conv3x3_ws<16, 16, 4, 0>", 2048): 292.1e6}
+                     ("k_conv3x3<64, 64, false, 4, 2, false>", 2048): 37.33e6,
+                     ("k_conv3x3_ws<16, 16, 4, 0>", 2048): 292.1e6}
 
 
 def build_problem(n):
