@@ -37,6 +37,7 @@ void tile_diagonal(Plan& P, ElemSet& S, const double* u, double* diag, bool over
 struct KState {
   double gamma, gamma_prev, pAp, alpha, beta, rr, atol2, bb, rr0, rmax2, delta2;
   double dot_out;  // scratch result of dot / norm kernels
+  double rsq;      // flexible CG: sum of squares of the network's input, r scaled by D^-1/2
   int k, maxiter, done, reliable;  // contiguous: copied to the host as one block
 };
 

@@ -50,24 +50,30 @@ __device__ inline void sums_in(const double* partials, int nb, double (&out)[NV]
   }
 }
 
-__global__ void k_fcg_init(int64_t n, const double* b, double* x, double* r, double* pa, double* partials) {
-  double v[1] = {0.0};
+// partials of r.r and of the network input's sum of squares, (D^-1/2 r).(D^-1/2 r) (dsq = D^-1/2,
+// or null: no scaling), so that the network's input scale needs no pass of its own
+__global__ void k_fcg_init(int64_t n, const double* b, const double* dsq, double* x, double* r, double* pa,
+                           double* partials) {
+  double v[2] = {0.0, 0.0};
   for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     const double bi = b[i];
     x[i] = 0.0;
     r[i] = bi;
     pa[i] = 0.0;
     v[0] += bi * bi;
+    const double si = dsq ? bi * dsq[i] : bi;
+    v[1] += si * si;
   }
-  partials_out<kBS, 1>(v, partials);
+  partials_out<kBS, 2>(v, partials);
 }
 
 __global__ __launch_bounds__(kFinBS) void k_fcg_init_fin(KState* S, const double* partials, int nb, double tol,
                                                          double atol, int maxiter) {
-  double s[1];
-  sums_in<1>(partials, nb, s);
+  double s[2];
+  sums_in<2>(partials, nb, s);
   if (threadIdx.x == 0) {
     S->rr = s[0];
+    S->rsq = s[1];
     S->rr0 = s[0];
     S->bb = s[0];
     S->atol2 = fmax(tol * tol * s[0], atol * atol);
@@ -175,27 +181,30 @@ __global__ __launch_bounds__(kFinBS) void k_fcg_alpha_fin(KState* S, const doubl
   }
 }
 
-__global__ void k_fcg_update(int64_t n, const double* pa, const double* pb, const double* q, double* x, double* r,
-                             KState* S, double* partials) {
+__global__ void k_fcg_update(int64_t n, const double* pa, const double* pb, const double* q, const double* dsq,
+                             double* x, double* r, KState* S, double* partials) {
   if (S->done) return;
   const double alpha = S->alpha;
   const double* p = (S->k & 1) ? pa : pb;
-  double v[1] = {0.0};
+  double v[2] = {0.0, 0.0};
   for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     v[0] += ri * ri;
+    const double si = dsq ? ri * dsq[i] : ri;
+    v[1] += si * si;
   }
-  partials_out<kBS, 1>(v, partials);
+  partials_out<kBS, 2>(v, partials);
 }
 
 __global__ __launch_bounds__(kFinBS) void k_fcg_update_fin(KState* S, const double* partials, int nb) {
   if (S->done) return;
-  double s[1];
-  sums_in<1>(partials, nb, s);
+  double s[2];
+  sums_in<2>(partials, nb, s);
   if (threadIdx.x == 0) {
     S->rr = s[0];
+    S->rsq = s[1];
     S->k += 1;
     S->done = !(s[0] > S->atol2 && S->k < S->maxiter);
   }
@@ -219,7 +228,8 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   const int gv = grid_vec(n), gs = grid_chunks(P.n_chunks);
   net_set_diag_scaling(P, vals, sizeof(T) == 8 ? APX_FP64 : APX_FP32, st);  // network sees D^-1/2 A D^-1/2
   APX_HIP(hipMemsetAsync(q, 0, n * sizeof(double), st));
-  hipLaunchKernelGGL(k_fcg_init, dim3(gv), dim3(kBS), 0, st, n, b, x, r, pa, P.partials.p);
+  const double* dsq = P.net_dsq_set ? P.net_dsq.p : nullptr;  // the network's input scaling
+  hipLaunchKernelGGL(k_fcg_init, dim3(gv), dim3(kBS), 0, st, n, b, dsq, x, r, pa, P.partials.p);
   hipLaunchKernelGGL(k_fcg_init_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv, tol, atol, maxiter);
   APX_LAUNCH_CHECK();
   std::vector<EventPair> ev_net, ev_spmv;
@@ -236,7 +246,7 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   N.stop = &S->done;
   auto iteration = [&] {
     EventPair e0 = timing_begin(P, st);
-    net_input_from_reduced_img(P, N, r, 1, g_nodal, &S->rr, 0, st);
+    net_input_from_reduced_img(P, N, r, 1, g_nodal, &S->rsq, 0, st);  // its scale from the last update
     N.forward(st, false);
     hipLaunchKernelGGL(k_fcg_z, dim3(gv), dim3(kBS), 0, st, n, P.red2full.p, N.y.p, P.net_dsq.p,
                        N.jacobi_skip ? r : nullptr, N.scale.p, q, z, S, P.partials.p);
@@ -252,7 +262,7 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
       launch_kernel(k_fcg_spmv<T, int32_t>, dim3(gs), dim3(kSpmvBS), 0, st, n, P.n_chunks, P.sell_ptr.p,
                     (const int32_t*)P.sell_cols.p, vals, z, pa, pb, q, r, S, P.partials.p);
     hipLaunchKernelGGL(k_fcg_alpha_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gs);
-    hipLaunchKernelGGL(k_fcg_update, dim3(gv), dim3(kBS), 0, st, n, pa, pb, q, x, r, S, P.partials.p);
+    hipLaunchKernelGGL(k_fcg_update, dim3(gv), dim3(kBS), 0, st, n, pa, pb, q, dsq, x, r, S, P.partials.p);
     hipLaunchKernelGGL(k_fcg_update_fin, dim3(1), dim3(kFinBS), 0, st, S, P.partials.p, gv);
     APX_LAUNCH_CHECK();
     if (P.timing) {
@@ -265,6 +275,7 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   // iteration queued past convergence costs ~70 early-exit launches
   constexpr int kBatch = 1;
   volatile char* pin = (volatile char*)P.pinned + 512;  // slot s: KState copy at byte 512 + 128 s
+  static_assert(sizeof(KState) <= 128, "KState must fit a 128-byte pinned slot");
   hipEvent_t evs[2];
   APX_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   APX_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
@@ -319,7 +330,8 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   if (hs.reliable == 2) res.stagnated = true;  // breakdown
   if (P.timing) {
     const size_t k = (size_t)hs.k;
-    const int64_t spmv_bytes = P.sell_total * (int64_t)(sizeof(T) + (P.sell_dcol.p ? 2 : 4)) + (P.n_chunks + 1) * 4 + n * 6 * 8;
+    const int64_t spmv_bytes =
+        P.sell_total * (int64_t)(sizeof(T) + (P.sell_dcol.p ? 2 : 4)) + (P.n_chunks + 1) * 4 + n * 6 * 8;
     timing_collect(P, "fcg_precond", ev_net, 0, k);
     timing_collect(P, sizeof(T) == 8 ? "fcg_spmv_f64" : "fcg_spmv_f32", ev_spmv, spmv_bytes, k);
     N.collect_timing(k);  // the forward passes of the executed iterations

@@ -1192,8 +1192,9 @@ void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, cons
   APX_REQUIRE(img >= 0 && img < N.batch, APX_ERR_ARG, "image index out of range");
   APX_REQUIRE(P.n_free > 0, APX_ERR_ARG, "network input: the plan has no free DOFs");
   ensure_krylov_scratch(P, st);
+  // the scale is the RMS of the network's input (r scaled by D^-1/2 when the plan has the
+  // scaling); rr_src, when given, is that input's sum of squares on the device
   const double* dsq = P.net_dsq_set ? P.net_dsq.p : nullptr;
-  if (dsq) rr_src = nullptr;  // the scale is the RMS of the scaled input
   int nb = 1;
   if (!rr_src) {
     grid1d(P.n_free, nb);

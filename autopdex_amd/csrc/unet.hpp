@@ -112,7 +112,8 @@ void net_set_diag_scaling(Plan& P, const void* vals_sell, int precision, hipStre
 
 // Preconditioner use: z = s * net(r / s) on the plan's free DOFs (grid = nodes),
 // with r and z scaled by D^-1/2 when the plan's diagonal scaling is set.
-// rr_src (device, optional) holds sum r^2 over the free DOFs; s = sqrt(rr / n_free).
+// s = sqrt(sum / n_free), the sum of squares of the network's input (r, scaled by D^-1/2 when
+// the scaling is set): read from rr_src (device) when given, else computed here.
 void net_input_from_reduced_img(Plan& P, UNet& N, const void* r, int r_f64, const double* g_nodal,
                                 const double* rr_src, int img, hipStream_t st);
 // r: the residual the input came from (FP64, reduced; used by the Jacobi skip, may be null
