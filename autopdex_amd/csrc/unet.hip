@@ -857,6 +857,40 @@ void conv_fwd_in_e0(UNet& N, hipStream_t st) {
     throw Error(APX_ERR_UNSUPPORTED, "fused input layer: 1 or 2 input channels");
 }
 
+// y = out(conv_d[0](s_0)) without storing conv_d's output (bit-identical to the two launches)
+void conv_fwd_d_out(UNet& N, hipStream_t st) {
+  const NetLayer& ld = N.layers[N.L_d[0]];
+  const NetLayer& lo = N.layers[N.L_out];
+  APX_REQUIRE(ld.ci == 16 && ld.co == 16 && lo.ci == 16 && (lo.co == 1 || lo.co == 2), APX_ERR_UNSUPPORTED,
+              "fused output layer: 16 -> 16 -> 1 or 2 channels");
+  ConvArgs a{};
+  a.x = N.s[0].p;
+  a.w = N.wf.p + ld.wf_off;
+  a.bias = N.theta.p + ld.b_off;
+  a.relu = ld.relu;
+  a.w2 = N.wf.p + lo.wf_off;
+  a.bias2 = N.theta.p + lo.b_off;
+  a.relu2 = lo.relu;
+  a.y = N.y.p;
+  a.yscale = nullptr;
+  a.B = N.batch;
+  a.H = N.H[0];
+  a.W = N.W[0];
+  a.stop = N.stop;
+  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 3) / (kTileW - 2)) * ((a.H + 3) / 4);
+  APX_REQUIRE(tiles < (int64_t(1) << 30) && (int64_t)a.H * a.W * 16 < INT32_MAX, APX_ERR_UNSUPPORTED,
+              "fused output layer: grid too large for 32-bit indices");
+  auto go = [&](auto kernel) {
+    static const int64_t cap = resident_blocks(kernel);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+    launch_kernel(kernel, dim3(g), dim3(kConvThreads), 0, st, a);
+  };
+  if (lo.co == 1)
+    go(k_conv_d_out<1>);
+  else
+    go(k_conv_d_out<2>);
+}
+
 void prolong_add(UNet& N, int level_fine, int C, const uint16_t* coarse, const uint16_t* skip, uint16_t* out,
                  hipStream_t st) {
   const int H = N.H[level_fine], W = N.W[level_fine];
@@ -1017,11 +1051,17 @@ void UNet::forward(hipStream_t st, bool keep_activations) {
            [&] { launch_kernel(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop); });
   else
     conv(L_b, L - 1, a[L - 1].p, u[L - 1].p);
+  // inference: the level-0 conv_d and the output layer in one launch (conv_d's output u_0 is read
+  // by nothing else; training keeps it for the backward pass)
+  const NetLayer& lo = layers[L_out];
+  const bool fused_out = fuse_tail && !keep_activations && L > 1 && ch[0] == 16 && layers[L_d[0]].ci == 16 &&
+                         layers[L_d[0]].co == 16 && lo.ci == 16 && (lo.co == 1 || lo.co == 2);
   for (int l = std::min(L - 2, c); l >= 0; --l) {
     if (l != c) {
       conv(L_up[l], l + 1, u[l + 1].p, t[l].p);
       transfer(false, l);
     }
+    if (l == 0 && fused_out) break;
     EventPair ec{};
     if (timed && l == 0) ec = timing_begin(*timer_plan, st);
     conv(L_d[l], l, s[l].p, u[l].p);
@@ -1030,7 +1070,12 @@ void UNet::forward(hipStream_t st, bool keep_activations) {
       ev_conv0.push_back(ec);
     }
   }
-  conv(L_out, 0, u[0].p, y.p);
+  if (fused_out)
+    record("k_conv_d_out<" + std::to_string(lo.co) + ">",
+           pixels(0) * (16 * 2 + lo.co * 4) + (int64_t)(layers[L_d[0]].cop * layers[L_d[0]].wk + lo.cop * lo.wk) * 2,
+           (int64_t)18 * 16 * (16 + lo.co) * pixels(0), [&] { conv_fwd_d_out(*this, st); });
+  else
+    conv(L_out, 0, u[0].p, y.p);
   APX_LAUNCH_CHECK();
   if (timed) {
     timing_end(*timer_plan, ef, st);

@@ -78,6 +78,11 @@ struct ConvArgs {
   const uint16_t* win;
   const float* bin;
   int relu_in;
+  // k_conv_d_out: the output layer applied to this convolution's output (weights as k_conv_out
+  // reads them: [co][9 taps x 16 channels], bias, ReLU); y is then the output layer's FP32 image
+  const uint16_t* w2;
+  const float* bias2;
+  int relu2;
 };
 
 // ---- forward / backward-data convolution ------------------------------------
@@ -722,6 +727,166 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a) {
       v += bias[co];
       if (a.relu) v = fmaxf(v, 0.f);
       dst[co] = v * scale;
+    }
+  }
+}
+
+// ---- level-0 conv_d fused with the output layer (inference) ---------------------------
+// y = conv_out(conv_d(s)) without storing conv_d's output: each block computes conv_d (16 -> 16,
+// the arithmetic of k_conv3x3_ws<16, 16, TR + 2, 0>) on the DR = TR + 2 rows x 64 columns around
+// its TR x 62 output tile, keeps that bf16 tile in LDS (zero outside the image, which is what
+// k_conv_out reads there), then forms k_conv_out's per-tap partials from it by the same MFMA and
+// sums them in the same order.  The result is bit-identical to the two launches; the 134 MB
+// intermediate image is neither written nor read back.  Tiles overlap by two columns.
+template <int CO>
+__global__ __launch_bounds__(kConvThreads) void k_conv_d_out(ConvArgs a) {
+  constexpr int TR = 4, DR = TR + 2;          // output rows, conv_d rows
+  constexpr int OW = kTileW - 2;              // output columns per tile
+  constexpr int TWP = kTileW + 2;             // staged row length (pixels)
+  constexpr int NP = (DR + 2) * TWP;          // staged pixels
+  constexpr int PP = ws_plane<DR>();          // pixels per plane
+  static_assert(PP >= NP, "plane holds the staged tile");
+  constexpr int NL = (NP + 15) / 16 * 16 * 2;  // 16-byte load slots (two per pixel)
+  constexpr int RC = (NL + kConvThreads - 1) / kConvThreads;
+  constexpr int NN = 9 * CO, ZS = NN + 1;     // output-layer partial columns; LDS floats per node
+  constexpr int NT = (NN + 15) / 16;          // their MFMA N-tiles
+  constexpr int SXE = 4 * PP * 4;             // staged tile (bf16 elements)
+  constexpr int SZE = DR * kTileW * ZS * 2;   // partials (bf16-element units: 2 per float)
+  // the partials reuse the staged tile's space (it is dead once conv_d's MFMAs are done)
+  __shared__ __attribute__((aligned(16))) uint16_t sx[SXE > SZE ? SXE : SZE];
+  __shared__ __attribute__((aligned(16))) uint16_t sd[DR * kTileW * 16];  // conv_d output tile
+  float* sz = (float*)sx;
+  if (a.stop && *a.stop) return;  // uniform over the grid: set before the launch
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  bf16x4 wa[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) wa[t] = *(const bf16x4*)(a.w + (int64_t)lr * 144 + t * 16 + 4 * lq);
+  float bb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bb[j] = a.bias ? a.bias[4 * lq + j] : 0.f;
+  // output layer: B fragments as in k_conv_out (column n = 9 co + t), bias
+  bf16x4 wb[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int n = min(16 * nt + lr, NN - 1);
+    wb[nt] = *(const bf16x4*)(a.w2 + (n / 9) * 144 + (n % 9) * 16 + 4 * lq);
+    if (16 * nt + lr >= NN) wb[nt] = bf16x4{0, 0, 0, 0};
+  }
+  float b2[CO];
+#pragma unroll
+  for (int co = 0; co < CO; ++co) b2[co] = a.bias2 ? a.bias2[co] : 0.f;
+
+  const int tiles_x = (a.W + OW - 1) / OW, tiles_y = (a.H + TR - 1) / TR;
+  const int tiles_img = tiles_x * tiles_y;
+  const int ntiles = a.B * tiles_img;  // the launcher checks that tile indices fit in 32 bits
+  // load slots as in k_conv3x3_ws (two 16-byte chunks per pixel, planar in channel quads)
+  int s_row[RC], s_col[RC], s_off[RC], s_lds[RC];
+#pragma unroll
+  for (int r = 0; r < RC; ++r) {
+    const int i = threadIdx.x + r * kConvThreads;
+    const int g = i >> 4, h = g % 2, pix = (g / 2) * 16 + (i & 15);
+    const bool used = i < NL && pix < NP;
+    s_row[r] = used ? pix / TWP : -(1 << 20);
+    s_col[r] = pix % TWP;
+    s_off[r] = ((pix / TWP) * a.W + pix % TWP) * 16 + 8 * h;
+    s_lds[r] = used ? ((2 * h) * PP + pix) * 4 : -1;
+  }
+  u32x4 qs[RC];
+  // staged origin: row ty0 - 2, column tx0 - 2 (conv_d rows ty0 - 1 .., columns tx0 - 1 ..)
+  auto load_tile = [&](int tile) {
+    const int img = tile / tiles_img, trem = tile - img * tiles_img;
+    const int tyi = trem / tiles_x;
+    const int ty0 = tyi * TR, tx0 = (trem - tyi * tiles_x) * OW;
+    const int64_t origin = ((int64_t)img * a.H * a.W + (int64_t)(ty0 - 2) * a.W + (tx0 - 2)) * 16;
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      const int yy = ty0 - 2 + s_row[r], xx = tx0 - 2 + s_col[r];
+      u32x4 q = u32x4{0u, 0u, 0u, 0u};
+      if ((unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W) q = *(const u32x4*)(a.x + origin + s_off[r]);
+      qs[r] = q;
+    }
+  };
+  const int tile0 = (int)xcd_block(blockIdx.x, gridDim.x);  // vertically adjacent tiles share an L2
+  if (tile0 < ntiles) load_tile(tile0);
+  for (int tile = tile0; tile < ntiles; tile += gridDim.x) {
+    const int img = tile / tiles_img, trem = tile - img * tiles_img;
+    const int tyi = trem / tiles_x;
+    const int ty0 = tyi * TR, tx0 = (trem - tyi * tiles_x) * OW;
+    __syncthreads();  // every wavefront is done with the previous tile's partials (sx)
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      if (s_lds[r] >= 0) {
+        *(uint2*)(sx + s_lds[r]) = uint2{qs[r][0], qs[r][1]};
+        *(uint2*)(sx + s_lds[r] + PP * 4) = uint2{qs[r][2], qs[r][3]};
+      }
+    }
+    __syncthreads();
+    if (tile + (int)gridDim.x < ntiles) load_tile(tile + (int)gridDim.x);  // in flight during the MFMAs
+    // conv_d: wavefront w owns conv_d columns 16 w .. 16 w + 15 of all DR rows
+    f32x4 acc[DR];
+#pragma unroll
+    for (int r = 0; r < DR; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int yi = 0; yi < DR + 2; ++yi)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const bf16x4 f = *(const bf16x4*)(sx + (lq * PP + yi * TWP + 16 * wave + lr + dx) * 4);
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const int r = yi - dy;  // conv_d row fed by staged row yi through tap row dy
+          if (r >= 0 && r < DR) acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[dy * 3 + dx], f, acc[r], 0, 0, 0);
+        }
+      }
+    // conv_d's output tile -> LDS (lane: channels 4 lq .. 4 lq + 3 of column 16 w + lr), zero outside
+    {
+      const int xx = tx0 - 1 + 16 * wave + lr;
+#pragma unroll
+      for (int r = 0; r < DR; ++r) {
+        const int yy = ty0 - 1 + r;
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          v[j] = acc[r][j] + bb[j];
+          if (a.relu) v[j] = fmaxf(v[j], 0.f);
+        }
+        const bool inside = (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+        *(uint2*)(sd + ((r * kTileW + 16 * wave + lr) * 16 + 4 * lq)) =
+            inside ? uint2{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])} : uint2{0u, 0u};
+      }
+    }
+    __syncthreads();  // the conv_d tile is complete and sx is free
+    // output-layer partials of the DR x 64 nodes: group g = (row g / 4, columns 16 (g % 4) ..)
+#pragma unroll
+    for (int k = 0; k < DR; ++k) {
+      const int g = wave + 4 * k, gr = g / 4, gc = 16 * (g % 4);
+      const bf16x4 af = *(const bf16x4*)(sd + ((gr * kTileW + gc + lr) * 16 + 4 * lq));
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, wb[nt], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        const int n = 16 * nt + lr;  // D: node 4 lq + j of the group, column n
+        if (n < NN) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) sz[(gr * kTileW + gc + 4 * lq + j) * ZS + n] = d[j];
+        }
+      }
+    }
+    __syncthreads();
+    // y: thread (r, x) sums the nine partials around conv_d node (r + 1, x + 1) in tap order
+    const int r = threadIdx.x >> 6, x = threadIdx.x & 63;
+    const int yy = ty0 + r, xx = tx0 + x;
+    if (x < OW && yy < a.H && xx < a.W) {
+      const float sc = a.yscale ? a.yscale[img] : 1.f;
+      float* dst = (float*)a.y + (((int64_t)img * a.H + yy) * a.W + xx) * CO;
+#pragma unroll
+      for (int co = 0; co < CO; ++co) {
+        float v = 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) v += sz[((r + t / 3) * kTileW + x + t % 3) * ZS + 9 * co + t];
+        v += b2[co];
+        if (a.relu2) v = fmaxf(v, 0.f);
+        dst[co] = v * sc;
+      }
     }
   }
 }
