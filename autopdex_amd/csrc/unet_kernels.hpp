@@ -739,7 +739,7 @@ __global__ __launch_bounds__(256) void k_conv_out(ConvArgs a) {
 // sums them in the same order.  The result is bit-identical to the two launches; the 134 MB
 // intermediate image is neither written nor read back.  Tiles overlap by two columns.
 template <int CO>
-__global__ __launch_bounds__(kConvThreads) void k_conv_d_out(ConvArgs a) {
+__global__ __launch_bounds__(kConvThreads, 4) void k_conv_d_out(ConvArgs a) {
   constexpr int TR = 4, DR = TR + 2;          // output rows, conv_d rows
   constexpr int OW = kTileW - 2;              // output columns per tile
   constexpr int TWP = kTileW + 2;             // staged row length (pixels)
@@ -780,18 +780,15 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_d_out(ConvArgs a) {
   const int tiles_x = (a.W + OW - 1) / OW, tiles_y = (a.H + TR - 1) / TR;
   const int tiles_img = tiles_x * tiles_y;
   const int ntiles = a.B * tiles_img;  // the launcher checks that tile indices fit in 32 bits
-  // load slots as in k_conv3x3_ws (two 16-byte chunks per pixel, planar in channel quads)
-  int s_row[RC], s_col[RC], s_off[RC], s_lds[RC];
-#pragma unroll
-  for (int r = 0; r < RC; ++r) {
-    const int i = threadIdx.x + r * kConvThreads;
-    const int g = i >> 4, h = g % 2, pix = (g / 2) * 16 + (i & 15);
-    const bool used = i < NL && pix < NP;
-    s_row[r] = used ? pix / TWP : -(1 << 20);
-    s_col[r] = pix % TWP;
-    s_off[r] = ((pix / TWP) * a.W + pix % TWP) * 16 + 8 * h;
-    s_lds[r] = used ? ((2 * h) * PP + pix) * 4 : -1;
-  }
+  // load slots as in k_conv3x3_ws (two 16-byte chunks per pixel, planar in channel quads): slot
+  // i = threadIdx.x + r kConvThreads takes chunk h of staged pixel pix; computed where used, not
+  // kept in registers (that keeps the kernel within 128 registers: 4 waves per SIMD)
+  auto slot = [&](int r, int& h, int& pix, bool& used) __attribute__((always_inline)) {
+    const int i = (int)threadIdx.x + r * kConvThreads;
+    h = (i >> 4) & 1;
+    pix = ((i >> 5) << 4) + (i & 15);
+    used = i < NL && pix < NP;
+  };
   u32x4 qs[RC];
   // staged origin: row ty0 - 2, column tx0 - 2 (conv_d rows ty0 - 1 .., columns tx0 - 1 ..)
   auto load_tile = [&](int tile) {
@@ -801,9 +798,14 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_d_out(ConvArgs a) {
     const int64_t origin = ((int64_t)img * a.H * a.W + (int64_t)(ty0 - 2) * a.W + (tx0 - 2)) * 16;
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      const int yy = ty0 - 2 + s_row[r], xx = tx0 - 2 + s_col[r];
+      int h, pix;
+      bool used;
+      slot(r, h, pix, used);
+      const int row = pix / TWP, col = pix - row * TWP;
+      const int yy = ty0 - 2 + row, xx = tx0 - 2 + col;
       u32x4 q = u32x4{0u, 0u, 0u, 0u};
-      if ((unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W) q = *(const u32x4*)(a.x + origin + s_off[r]);
+      if (used && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W)
+        q = *(const u32x4*)(a.x + origin + ((int64_t)row * a.W + col) * 16 + 8 * h);
       qs[r] = q;
     }
   };
@@ -816,9 +818,13 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_d_out(ConvArgs a) {
     __syncthreads();  // every wavefront is done with the previous tile's partials (sx)
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-      if (s_lds[r] >= 0) {
-        *(uint2*)(sx + s_lds[r]) = uint2{qs[r][0], qs[r][1]};
-        *(uint2*)(sx + s_lds[r] + PP * 4) = uint2{qs[r][2], qs[r][3]};
+      int h, pix;
+      bool used;
+      slot(r, h, pix, used);
+      if (used) {
+        const int o = ((2 * h) * PP + pix) * 4;
+        *(uint2*)(sx + o) = uint2{qs[r][0], qs[r][1]};
+        *(uint2*)(sx + o + PP * 4) = uint2{qs[r][2], qs[r][3]};
       }
     }
     __syncthreads();
