@@ -191,15 +191,17 @@ __device__ __forceinline__ u32x4 bf8_pack(const float f[8]) {
 
 // One 8-channel chunk of out = skip + P(coarse) at fine node (i, j); pointers at the image,
 // C channels per node.
+// coarse node (I, J) at coarse + I * crs + J * cps, skip node (i, j) at skip + i * srs + j * sps
 template <int C>
-__device__ __forceinline__ u32x4 prolong_chunk(const uint16_t* __restrict__ coarse, const uint16_t* skip, int i,
-                                               int j, int c, int W, int Hc, int Wc) {
+__device__ __forceinline__ u32x4 prolong_chunk_s(const uint16_t* __restrict__ coarse, int crs, int cps,
+                                                 const uint16_t* skip, int srs, int sps, int i, int j, int c, int Hc,
+                                                 int Wc) {
   const int I0 = i >> 1, J0 = j >> 1;
   const int I1 = min((i & 1) ? I0 + 1 : I0, Hc - 1), J1 = min((j & 1) ? J0 + 1 : J0, Wc - 1);
   const uint16_t* cb = coarse + c;
   float v[8], t[8];
   auto acc = [&](int I, int J, float w) {
-    bf8_unpack(*(const u32x4*)(cb + (I * Wc + J) * C), t);
+    bf8_unpack(*(const u32x4*)(cb + I * crs + J * cps), t);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] += w * t[e];
   };
@@ -212,7 +214,7 @@ __device__ __forceinline__ u32x4 prolong_chunk(const uint16_t* __restrict__ coar
   // once; a missing term adds w * 0 = +0, which leaves v unchanged (v starts at +0 and a sum
   // of this form never becomes -0), so the bits match skipping it
   auto acc_if = [&](bool on, int I, int J, float w) {
-    bf8_unpack(*(const u32x4*)(cb + (I * Wc + J) * C), t);
+    bf8_unpack(*(const u32x4*)(cb + I * crs + J * cps), t);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] += w * (on ? t[e] : 0.f);
   };
@@ -221,11 +223,17 @@ __device__ __forceinline__ u32x4 prolong_chunk(const uint16_t* __restrict__ coar
   acc_if(hi, I1, J0, 0.5f * wj0);
   acc_if(hi && hj, I1, J1, 0.25f);
   if (skip) {
-    bf8_unpack(*(const u32x4*)(skip + (i * W + j) * C + c), t);
+    bf8_unpack(*(const u32x4*)(skip + i * srs + j * sps + c), t);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] += t[e];
   }
   return bf8_pack(v);
+}
+
+template <int C>
+__device__ __forceinline__ u32x4 prolong_chunk(const uint16_t* __restrict__ coarse, const uint16_t* skip, int i,
+                                               int j, int c, int W, int Hc, int Wc) {
+  return prolong_chunk_s<C>(coarse, Wc * C, C, skip, W * C, C, i, j, c, Hc, Wc);
 }
 
 // out = skip + P(coarse), one thread per coarse node and 8-channel chunk, writing the (up to)
@@ -304,8 +312,12 @@ __global__ __launch_bounds__(256) void k_prolong_add(const uint16_t* __restrict_
 
 // R = P^T: coarse(I, J) = sum_{di,dj in -1..1} w(di) w(dj) fine(2I+di, 2J+dj); one 8-channel
 // chunk at coarse node (I, J), pointer at the image.
+// Node (i, j) of a tensor at base + i * rs + j * ps (elements): rs = W C, ps = C for the HBM
+// tensors; the coarse-level launch's padded LDS buffers have rs = (W + 2) kTailS, ps = kTailS,
+// base past their one-node border.
 template <int C>
-__device__ __forceinline__ u32x4 restrict_chunk(const uint16_t* __restrict__ fine, int I, int J, int c, int H, int W) {
+__device__ __forceinline__ u32x4 restrict_chunk_s(const uint16_t* __restrict__ fine, int I, int J, int c, int H, int W,
+                                                  int rs, int ps) {
   const uint16_t* fb = fine + c;
   float v[8], t[8];
 #pragma unroll
@@ -321,11 +333,16 @@ __device__ __forceinline__ u32x4 restrict_chunk(const uint16_t* __restrict__ fin
       const int ic = min(max(i, 0), H - 1), jc = min(max(j, 0), W - 1);
       const bool in = i == ic && j == jc;
       const float w = (di ? 0.5f : 1.f) * (dj ? 0.5f : 1.f);
-      bf8_unpack(*(const u32x4*)(fb + (ic * W + jc) * C), t);
+      bf8_unpack(*(const u32x4*)(fb + ic * rs + jc * ps), t);
 #pragma unroll
       for (int e = 0; e < 8; ++e) v[e] += w * (in ? t[e] : 0.f);
     }
   return bf8_pack(v);
+}
+
+template <int C>
+__device__ __forceinline__ u32x4 restrict_chunk(const uint16_t* __restrict__ fine, int I, int J, int c, int H, int W) {
+  return restrict_chunk_s<C>(fine, I, J, c, H, W, W * C, C);
 }
 
 template <int C8>
@@ -340,34 +357,35 @@ __global__ __launch_bounds__(256) void k_restrict(const uint16_t* __restrict__ f
 }
 
 // ---- coarse-level tail of the forward pass ----------------------------------------------
-// One workgroup per image runs the steps in order, with a block barrier after each (the
-// workgroup's global stores are visible to all its threads after the barrier).
+// One workgroup per image runs the steps in order, with a block barrier after each.  The
+// activations of the coarse levels stay in LDS for the whole launch: each level has up to three
+// buffers in a padded layout ((H + 2) x (W + 2) nodes, kTailS elements per node, zero border),
+// so a convolution reads its 3 x 3 windows straight from its input buffer and writes its output
+// into the interior of another.  Only the first step (the restriction of level c, from HBM) and
+// the last (the prolongation into level c, to HBM) touch global memory.
 // Convolution steps form the same products in the same accumulation order as
 // k_conv3x3<64, 64, false, NB> (A = 16 pixels x 32 K, B = 32 K x 16 output channels, K
-// blocks tap-major, 32 channels each, on the 16x16x32 MFMA), with the same epilogue, so the
-// results are bit-identical to the per-layer launches.  Pixels are grouped 16 at a time in row-major
-// order over the whole grid; the input is staged in LDS with a zero halo ((H + 2) x (W + 2)
-// nodes, pixel stride kTailS), the weights as [co][k] rows of stride kTailSW.  Wavefront w
+// blocks tap-major, 32 channels each, on the 16x16x32 MFMA), with the same epilogue, and the
+// transfers use the arithmetic of k_restrict / the prolongation, so the results are
+// bit-identical to the per-layer launches.  Pixels are grouped 16 at a time in row-major order
+// over the whole grid; the weights are staged as [co][k] rows of stride kTailSW.  Wavefront w
 // computes output channels 32 (w & 1) .. 32 (w & 1) + 31 of the pixel groups w / 2,
 // w / 2 + 4, ...
 // Weights arrive with 16-byte loads spread over the whole workgroup (a wavefront gathering
 // its own B fragments from global memory touches 16 cache lines per load, which on one
 // compute unit costs far more than the matrix work).  The next convolution's weights are
 // loaded into registers while the current step computes: they are issued after the step's
-// bias and input loads (loads complete in issue order as far as waits go), unconditionally
-// (the last convolution reloads its own), and the barrier between staging and compute waits
-// for LDS only.
+// bias loads (loads complete in issue order as far as waits go), unconditionally (the last
+// convolution reloads its own), and the barrier between the weight stores and the matrix work
+// waits for LDS only.
 constexpr int kTailThreads = 512;
-constexpr int kTailS = 72;
-constexpr int kTailSW = 576 + 8;
-constexpr int kTailPix = (kTailMaxGrid + 2) * (kTailMaxGrid + 2);
-constexpr int kTailStage = (kTailPix * 8 + kTailThreads - 1) / kTailThreads;  // staged input chunks per thread
-constexpr int kTailWChunks = 64 * 576 / 8 / kTailThreads;                      // weight chunks per thread
+constexpr int kTailWChunks = 64 * 576 / 8 / kTailThreads;  // weight chunks per thread
 static_assert(64 * 576 / 8 % kTailThreads == 0, "weight chunks must tile the workgroup");
 
 template <int NM>
 __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint16_t* sw, int m0, int H, int W, int cp,
-                                                 float bias0, float bias1, int relu, uint16_t* Y, int lr, int lg) {
+                                                 float bias0, float bias1, int relu, uint16_t* Y, uint16_t* Yg, int lr,
+                                                 int lg) {
   const int WP = W + 2, HW = H * W;
   int lp[NM];
 #pragma unroll
@@ -392,6 +410,8 @@ __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint1
       acc[mm][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b1, acc[mm][1], 0, 0, 0);
     }
   }
+  // output: node p of the grid -> interior node (p / W + 1, p % W + 1) of the output buffer, and
+  // (training, Yg set) node p of the HBM tensor, which the backward pass reads
 #pragma unroll
   for (int n = 0; n < 2; ++n) {
     const int co = 32 * cp + 16 * n + lr;
@@ -403,14 +423,19 @@ __device__ __forceinline__ void tail_conv_groups(const uint16_t* sx, const uint1
         const int p = (m0 + 4 * mm) * 16 + 4 * lg + j;
         float v = acc[mm][n][j] + bb;
         if (relu) v = fmaxf(v, 0.f);
-        if (p < HW) Y[p * 64 + co] = f2bf(v);
+        if (p < HW) {
+          const uint16_t h = f2bf(v);
+          Y[((p / W + 1) * WP + p % W + 1) * kTailS + co] = h;
+          if (Yg) Yg[p * 64 + co] = h;
+        }
       }
   }
 }
 
-__global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const int* stop) {
+// store: also write every step's output to its HBM tensor (training: the backward pass reads them)
+__global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const int* stop, int store) {
   if (stop && *stop) return;  // uniform over the grid: set before the launch
-  __shared__ __attribute__((aligned(16))) uint16_t sx[kTailPix * kTailS];
+  __shared__ __attribute__((aligned(16))) uint16_t act[kTailPool];
   __shared__ __attribute__((aligned(16))) uint16_t sw[64 * kTailSW];
   const int img = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
@@ -424,79 +449,104 @@ __global__ __launch_bounds__(kTailThreads) void k_unet_tail(TailArgs args, const
     }
   };
   fetch_weights(args.w_first);
+  // zero pool: the buffers' borders stay zero (steps write interiors only)
+  for (int i = 8 * (int)threadIdx.x; i < args.pool; i += 8 * kTailThreads) *(u32x4*)(act + i) = u32x4{0u, 0u, 0u, 0u};
+  __syncthreads();
   for (int si = 0; si < args.n; ++si) {
     const TailStep& st = args.step[si];
     if (st.op == kTailConv) {
-      const int H = st.H, W = st.W, WP = W + 2, NP8 = (H + 2) * WP * 8;
-      const int64_t io = (int64_t)img * H * W * 64;
-      const uint16_t* X = st.x + io;
-      // the prefetched weights -> LDS (every reader of the previous step passed its barrier)
+      const int H = st.H, W = st.W;
+      // the prefetched weights -> LDS (every reader of the previous convolution passed its barrier)
 #pragma unroll
       for (int r = 0; r < kTailWChunks; ++r) {
         const int i = threadIdx.x + r * kTailThreads;
         *(u32x4*)(sw + (i / 72) * kTailSW + (i % 72) * 8) = qw[r];
       }
       const float bias0 = st.bias[32 * cp + lr], bias1 = st.bias[32 * cp + 16 + lr];
-      u32x4 q[kTailStage];
-#pragma unroll
-      for (int r = 0; r < kTailStage; ++r) {
-        const int i = threadIdx.x + r * kTailThreads;
-        const int pp = i >> 3, py = pp / WP - 1, px = pp % WP - 1;
-        q[r] = u32x4{0u, 0u, 0u, 0u};
-        if (i < NP8 && py >= 0 && py < H && px >= 0 && px < W)
-          q[r] = *(const u32x4*)(X + (py * W + px) * 64 + (i & 7) * 8);
-      }
       fetch_weights(st.w_next);  // in flight through this step's matrix work
-#pragma unroll
-      for (int r = 0; r < kTailStage; ++r) {
-        const int i = threadIdx.x + r * kTailThreads;
-        if (i < NP8) *(u32x4*)(sx + (i >> 3) * kTailS + (i & 7) * 8) = q[r];
-      }
       lds_barrier();
-      uint16_t* Y = st.y + io;
+      const uint16_t* X = act + st.xo;
+      uint16_t* Y = act + st.yo;
+      uint16_t* Yg = store ? st.y + (int64_t)img * H * W * 64 : nullptr;
       const int NG = (H * W + 15) / 16;
       for (int m0 = ph; m0 < NG; m0 += 16) {
         switch (min(4, (NG - m0 + 3) / 4)) {
-          case 4: tail_conv_groups<4>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
-          case 3: tail_conv_groups<3>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
-          case 2: tail_conv_groups<2>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
-          default: tail_conv_groups<1>(sx, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, lr, lg); break;
+          case 4: tail_conv_groups<4>(X, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, Yg, lr, lg); break;
+          case 3: tail_conv_groups<3>(X, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, Yg, lr, lg); break;
+          case 2: tail_conv_groups<2>(X, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, Yg, lr, lg); break;
+          default: tail_conv_groups<1>(X, sw, m0, H, W, cp, bias0, bias1, st.relu, Y, Yg, lr, lg); break;
         }
       }
     } else if (st.op == kTailRestrict) {
+      // fine grid H x W (LDS buffer, or level c in HBM) -> coarse Hc x Wc (LDS buffer interior)
       const int n8 = st.Hc * st.Wc * 8;
-      const uint16_t* F = st.x + (int64_t)img * st.H * st.W * 64;
-      uint16_t* Y = st.y + (int64_t)img * st.Hc * st.Wc * 64;
-      for (int i0 = threadIdx.x; i0 < n8; i0 += 4 * kTailThreads) {
-        u32x4 r[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u * kTailThreads, node = i >> 3;
-          r[u] = i < n8 ? restrict_chunk<64>(F, node / st.Wc, node % st.Wc, (i & 7) * 8, st.H, st.W)
-                        : u32x4{0u, 0u, 0u, 0u};
+      const int crs = (st.Wc + 2) * kTailS;
+      uint16_t* Yc = act + st.yo + crs + kTailS;
+      uint16_t* Yg = store ? st.y + (int64_t)img * st.Hc * st.Wc * 64 : nullptr;
+      if (st.xo >= 0) {
+        const int frs = (st.W + 2) * kTailS;
+        const uint16_t* F = act + st.xo + frs + kTailS;
+        for (int i = threadIdx.x; i < n8; i += kTailThreads) {
+          const int node = i >> 3, I = node / st.Wc, J = node % st.Wc;
+          const u32x4 r = restrict_chunk_s<64>(F, I, J, (i & 7) * 8, st.H, st.W, frs, kTailS);
+          *(u32x4*)(Yc + I * crs + J * kTailS + (i & 7) * 8) = r;
+          if (Yg) *(u32x4*)(Yg + node * 64 + (i & 7) * 8) = r;
         }
+      } else {
+        const uint16_t* F = st.x + (int64_t)img * st.H * st.W * 64;
+        for (int i0 = threadIdx.x; i0 < n8; i0 += 4 * kTailThreads) {
+          u32x4 r[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u * kTailThreads;
-          if (i < n8) *(u32x4*)(Y + (i >> 3) * 64 + (i & 7) * 8) = r[u];
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kTailThreads, node = i >> 3;
+            r[u] = i < n8 ? restrict_chunk<64>(F, node / st.Wc, node % st.Wc, (i & 7) * 8, st.H, st.W)
+                          : u32x4{0u, 0u, 0u, 0u};
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kTailThreads, node = i >> 3;
+            if (i < n8) {
+              *(u32x4*)(Yc + (node / st.Wc) * crs + (node % st.Wc) * kTailS + (i & 7) * 8) = r[u];
+              if (Yg) *(u32x4*)(Yg + node * 64 + (i & 7) * 8) = r[u];
+            }
+          }
         }
       }
     } else {
+      // coarse grid Hc x Wc (LDS buffer) + skip -> fine grid H x W (LDS buffer interior, or level c
+      // in HBM with its skip from HBM)
       const int n8 = st.H * st.W * 8;
-      const int64_t io = (int64_t)img * st.H * st.W * 64;
-      const uint16_t* Cg = st.x + (int64_t)img * st.Hc * st.Wc * 64;
-      for (int i0 = threadIdx.x; i0 < n8; i0 += 4 * kTailThreads) {
-        u32x4 r[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u * kTailThreads, node = i >> 3;
-          r[u] = i < n8 ? prolong_chunk<64>(Cg, st.skip + io, node / st.W, node % st.W, (i & 7) * 8, st.W, st.Hc, st.Wc)
-                        : u32x4{0u, 0u, 0u, 0u};
+      const int crs = (st.Wc + 2) * kTailS;
+      const uint16_t* Cg = act + st.xo + crs + kTailS;
+      if (st.yo >= 0) {
+        const int frs = (st.W + 2) * kTailS;
+        const uint16_t* Sk = act + st.so + frs + kTailS;
+        uint16_t* Yf = act + st.yo + frs + kTailS;
+        uint16_t* Yg = store ? st.y + (int64_t)img * st.H * st.W * 64 : nullptr;
+        for (int i = threadIdx.x; i < n8; i += kTailThreads) {
+          const int node = i >> 3, fi = node / st.W, fj = node % st.W;
+          const u32x4 r = prolong_chunk_s<64>(Cg, crs, kTailS, Sk, frs, kTailS, fi, fj, (i & 7) * 8, st.Hc, st.Wc);
+          *(u32x4*)(Yf + fi * frs + fj * kTailS + (i & 7) * 8) = r;
+          if (Yg) *(u32x4*)(Yg + node * 64 + (i & 7) * 8) = r;
         }
+      } else {
+        const int64_t io = (int64_t)img * st.H * st.W * 64;
+        const uint16_t* Sk = st.skip + io;
+        uint16_t* Yg = st.y + io;
+        for (int i0 = threadIdx.x; i0 < n8; i0 += 4 * kTailThreads) {
+          u32x4 r[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u * kTailThreads;
-          if (i < n8) *(u32x4*)(st.y + io + (i >> 3) * 64 + (i & 7) * 8) = r[u];
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kTailThreads, node = i >> 3;
+            r[u] = i < n8 ? prolong_chunk_s<64>(Cg, crs, kTailS, Sk, st.W * 64, 64, node / st.W, node % st.W,
+                                                (i & 7) * 8, st.Hc, st.Wc)
+                          : u32x4{0u, 0u, 0u, 0u};
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kTailThreads;
+            if (i < n8) *(u32x4*)(Yg + (i >> 3) * 64 + (i & 7) * 8) = r[u];
+          }
         }
       }
     }
@@ -711,44 +761,67 @@ void UNet::build_tail() {
   }
   if (tail_level < 0) return;
   const int c = tail_level;
-  auto conv = [&](int li, int level, const uint16_t* x, uint16_t* y) {
+  // LDS buffers: level l in (c, L) has X_l, Y_l and (l < L - 1) the skip buffer D_l, each
+  // (H + 2) x (W + 2) nodes of kTailS elements; an offset of -1 stands for the HBM tensor
+  int X[kMaxLevels], Y[kMaxLevels], D[kMaxLevels];
+  int pool = 0;
+  for (int l = c + 1; l < L; ++l) {
+    const int sz = (H[l] + 2) * (W[l] + 2) * kTailS;  // a multiple of 8
+    X[l] = pool;
+    Y[l] = pool + sz;
+    D[l] = l < L - 1 ? pool + 2 * sz : -1;
+    pool += (l < L - 1 ? 3 : 2) * sz;
+  }
+  if (pool > kTailPool) {  // the coarse levels do not fit in LDS: per-layer launches
+    tail_level = -1;
+    tail = TailArgs{};
+    return;
+  }
+  tail.pool = pool;
+  auto conv = [&](int li, int level, int xo, uint16_t* y, int yo) {
     const NetLayer& ly = layers[li];
     TailStep& ts = tail.step[tail.n++];
     ts.op = kTailConv;
-    ts.x = x;
     ts.y = y;
+    ts.xo = xo;
+    ts.yo = yo;
+    ts.so = -1;
     ts.w = wf.p + ly.wf_off;
     ts.bias = theta.p + ly.b_off;
     ts.relu = ly.relu;
     ts.H = H[level];
     ts.W = W[level];
   };
-  auto transfer = [&](int op, int l, const uint16_t* x, const uint16_t* skip, uint16_t* y) {
+  auto transfer = [&](int op, int l, const uint16_t* x, int xo, const uint16_t* skip, int so, uint16_t* y, int yo) {
     TailStep& ts = tail.step[tail.n++];
     ts.op = op;
     ts.x = x;
+    ts.xo = xo;
     ts.skip = skip;
+    ts.so = so;
     ts.y = y;
+    ts.yo = yo;
     ts.H = H[l];
     ts.W = W[l];
     ts.Hc = H[l + 1];
     ts.Wc = W[l + 1];
   };
-  transfer(kTailRestrict, c, d[c].p, nullptr, rd[c].p);
-  conv(L_down[c], c + 1, rd[c].p, a[c + 1].p);
+  // (each step's HBM tensor is also named: training writes it for the backward pass)
+  transfer(kTailRestrict, c, d[c].p, -1, nullptr, -1, rd[c].p, X[c + 1]);      // rd_c -> X_{c+1}
+  conv(L_down[c], c + 1, X[c + 1], a[c + 1].p, Y[c + 1]);                     // a_{c+1} -> Y_{c+1}
   for (int l = c + 1; l + 1 < L; ++l) {
-    conv(L_e[l], l, a[l].p, d[l].p);
-    transfer(kTailRestrict, l, d[l].p, nullptr, rd[l].p);
-    conv(L_down[l], l + 1, rd[l].p, a[l + 1].p);
+    conv(L_e[l], l, Y[l], d[l].p, D[l]);                                      // d_l -> D_l
+    transfer(kTailRestrict, l, nullptr, D[l], nullptr, -1, rd[l].p, X[l + 1]);  // rd_l -> X_{l+1}
+    conv(L_down[l], l + 1, X[l + 1], a[l + 1].p, Y[l + 1]);                   // a_{l+1} -> Y_{l+1}
   }
-  conv(L_b, L - 1, a[L - 1].p, u[L - 1].p);
+  conv(L_b, L - 1, Y[L - 1], u[L - 1].p, X[L - 1]);                           // u_{L-1} -> X_{L-1}
   for (int l = L - 2; l > c; --l) {
-    conv(L_up[l], l + 1, u[l + 1].p, t[l].p);
-    transfer(kTailProlong, l, t[l].p, d[l].p, s[l].p);
-    conv(L_d[l], l, s[l].p, u[l].p);
+    conv(L_up[l], l + 1, X[l + 1], t[l].p, Y[l + 1]);                         // t_l -> Y_{l+1}
+    transfer(kTailProlong, l, nullptr, Y[l + 1], nullptr, D[l], s[l].p, Y[l]);  // s_l -> Y_l
+    conv(L_d[l], l, Y[l], u[l].p, X[l]);                                      // u_l -> X_l
   }
-  conv(L_up[c], c + 1, u[c + 1].p, t[c].p);
-  transfer(kTailProlong, c, t[c].p, d[c].p, s[c].p);
+  conv(L_up[c], c + 1, X[c + 1], t[c].p, Y[c + 1]);                           // t_c -> Y_{c+1}
+  transfer(kTailProlong, c, nullptr, Y[c + 1], d[c].p, -1, s[c].p, -1);       // s_c (HBM)
   APX_REQUIRE(tail.n == 6 * (L - 1 - c) - 1, APX_ERR_STATE, "coarse-level tail: step count");
   // each convolution prefetches the weights of the next one (the last one its own)
   const uint16_t* next = nullptr;
@@ -1048,7 +1121,9 @@ void UNet::forward(hipStream_t st, bool keep_activations) {
   }
   if (fused)
     record("k_unet_tail", 0, 0,
-           [&] { launch_kernel(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop); });
+           [&] {
+             launch_kernel(k_unet_tail, dim3(batch), dim3(kTailThreads), 0, st, tail, stop, keep_activations ? 1 : 0);
+           });
   else
     conv(L_b, L - 1, a[L - 1].p, u[L - 1].p);
   // inference: the level-0 conv_d and the output layer in one launch (conv_d's output u_0 is read

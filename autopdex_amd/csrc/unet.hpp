@@ -34,13 +34,23 @@ struct TailStep {
   int relu;
   int H, W;              // convolution grid; transfers: the fine grid
   int Hc, Wc;            // transfers: the coarse grid
+  // LDS element offsets of the input, skip and output buffers in the launch's activation pool
+  // (padded layout: (H + 2) x (W + 2) nodes, kTailS elements per node, zero border); -1: the
+  // tensor is in HBM at x / skip / y (the first step's input, the last step's skip and output)
+  int xo, so, yo;
 };
 constexpr int kTailMaxSteps = 28;  // 2 KB of kernel arguments
 constexpr int kTailMaxGrid = 9;    // largest convolution grid side in the tail
+// LDS of the coarse-level launch: the weights of one 64 -> 64 convolution (64 rows of 584
+// bf16) and a pool of kTailPool bf16 elements for the activations, 160 KB in all
+constexpr int kTailS = 72;        // LDS elements per node of an activation buffer (64 + 8)
+constexpr int kTailSW = 576 + 8;  // LDS elements per weight row
+constexpr int kTailPool = (160 * 1024 - 64 * kTailSW * 2) / 2;
 struct TailArgs {
   TailStep step[kTailMaxSteps];
   const uint16_t* w_first;  // weights of the first convolution step
   int n;
+  int pool;  // activation pool elements used (a multiple of 8)
 };
 
 struct UNet {
