@@ -3,9 +3,11 @@
 The network approximates A^-1 r on the structured node grid and is used
 inside flexible CG (static_settings['type of preconditioner'] = 'learned').
 Training data are (residual, error) pairs that the plain solver itself
-produces: Jacobi-PCG iterates x_k of solves A x = b give r_k = b - A x_k and
-e_k = x* - x_k.  Forward, backward and Adam run in libautopdex_amd's HIP
-kernels (bf16 MFMA convolutions, FP32 master weights).
+produces: iterates x_k of solves A x = b give r_k = b - A x_k and
+e_k = x* - x_k, with x* from Jacobi-PCG; the last part of fit() adds the
+iterates of flexible CG with the network being trained.  Forward, backward
+and Adam run in libautopdex_amd's HIP kernels (bf16 MFMA convolutions, FP32
+master weights).
 
 No counterpart in the reference; it plugs into the reference's Newton /
 linear-solve flow through the 'hip' backend's preconditioner option.
@@ -235,7 +237,8 @@ def default_levels(height, width, coarsest=3):
 
 def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=16, n_smooth=16,
         pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
-        log=None, time_budget=None, jacobi_skip=False):
+        log=None, time_budget=None, jacobi_skip=False, self_fraction=1.0 / 3.0, self_lr=None,
+        self_iterates=(4, 8, 16, 32)):
     """Train a linear U-Net preconditioner for `problem` (d DOFs per node on a structured
     height x width node grid: d input and output channels) and attach it.
 
@@ -243,7 +246,16 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     and smooth random right-hand sides r (squared-exponential fields of lengths 0.2 ... 0.005).
     The loss is the relative A-norm error, the quantity CG minimizes.  Every `validate_every`
     steps the parameters are scored by flexible CG (to 1e-6) on a held-out right-hand side and
-    the best ones are kept.  Returns (network with batch 1, attached; info dict)."""
+    the best ones are kept.
+
+    The last `self_fraction` of the steps also train on the residuals that flexible CG with the
+    network meets: the best network so far runs `self_iterates` iterations of flexible CG on
+    every training right-hand side b, and each iterate x_k adds the pair (b - A x_k, x* - x_k)
+    (x* from the plain solve).  These steps run at `self_lr` (default 0.15 lr), decaying to a
+    third of it.  At 2048^2 this took the benchmark's learned_mixed Newton solve from 48 to 33
+    flexible-CG iterations at the same number of steps; Jacobi-PCG iterates as extra pairs made
+    the network fail instead.  self_fraction = 0 trains on the plain solver's pairs only.
+    Returns (network with batch 1, attached; info dict)."""
     torch = torch_cuda()
     t0 = time.perf_counter()
     if vals is None:
@@ -269,6 +281,7 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     pairs = []
     for b in rhs:
         pairs += solver_pairs(problem, vals, b, checkpoints=(0,), tol=pair_tol)
+    n_plain = len(pairs)  # pairs[i] = (b_i, x*_i): the iterate x_0 = 0
     b_val = noise() + smooth(n_smooth + 1) * 10.0  # held out
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
@@ -283,10 +296,36 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     t1 = time.perf_counter()
     best, history, done = None, [], 0
     seg = validate_every if validate_every else steps
+    n_self = int(round(steps * self_fraction)) if self_iterates else 0
+    n_first = steps - n_self
+    lr_self = 0.15 * lr if self_lr is None else self_lr
+    n_self_pairs, t_self = 0, 0.0
     while done < steps:
-        k = min(seg, steps - done)
-        lr0 = lr * (0.1 ** (done / steps))
-        lr1 = lr * (0.1 ** ((done + k) / steps))
+        if done == n_first and n_self > 0:
+            # pairs from the flexible-CG iterates of the best network so far
+            ts = time.perf_counter()
+            inf.set_params(best[1])
+            net.set_params(best[1])
+            # (x* solves to pair_tol: an iterate whose residual is near that carries no signal)
+            for b, xs in pairs[:n_plain]:
+                for k_it in self_iterates:
+                    xk, _ = fcg(problem, vals, b, tol=0.0, maxiter=k_it)
+                    rk = b - spmv(problem, vals, xk)
+                    if bool(torch.isfinite(xk).all()) and float(rk.norm()) > 100.0 * pair_tol * float(b.norm()):
+                        pairs.append((rk, xs - xk))
+                        n_self_pairs += 1
+            torch.cuda.synchronize()
+            t_self = time.perf_counter() - ts
+            if log:
+                log(f"[fit] {n_self_pairs} pairs from flexible-CG iterates in {t_self:.1f} s")
+        if done < n_first:
+            k = min(seg, n_first - done)
+            lr0 = lr * (0.1 ** (done / steps))
+            lr1 = lr * (0.1 ** ((done + k) / steps))
+        else:
+            k = min(seg, steps - done)
+            lr0 = lr_self * (1.0 / 3.0) ** ((done - n_first) / n_self)
+            lr1 = lr_self * (1.0 / 3.0) ** ((done + k - n_first) / n_self)
         history += train(net, problem, pairs, vals, steps=k, lr=lr0, lr_final=lr1,
                          warmup=50 if done == 0 else 0, seed=seed + done, loss="anorm", log=None)
         done += k
@@ -304,7 +343,8 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     inf.set_params(best[1])
-    info = dict(pairs=len(pairs), pair_seconds=t_pairs, train_seconds=t_train, steps=done, best_step=best[2],
+    info = dict(pairs=n_plain, self_pairs=n_self_pairs, pair_seconds=t_pairs, self_pair_seconds=t_self,
+                train_seconds=t_train, steps=done, self_steps=max(0, done - n_first), best_step=best[2],
                 validation_iterations=best[0], levels=L, channels=list(ch), batch=batch,
                 final_loss=float(np.mean(history[-50:])), n_params=int(inf.n_params))
     return inf, info
