@@ -270,9 +270,13 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
       ev_spmv.push_back(e1);
     }
   };
-  // host polling: batches of kBatch iterations, flags read one batch late.  One iteration per
-  // batch: an iteration is ~1.5 ms of device work against ~0.3 ms to enqueue it, and every
-  // iteration queued past convergence costs ~70 early-exit launches
+  // host polling: one iteration per batch, its state copied to a pinned slot and read while
+  // the next one runs (an iteration is ~1 ms of device work against ~0.3 ms to enqueue it).
+  // An iteration queued past convergence costs ~70 early-exit launches (~0.3 ms), once per
+  // inner solve, so when the last read residual, extrapolated by its last ratio, says that
+  // the queued iteration converges, nothing more is queued until it is read; a wrong guess
+  // costs the host's wake-up latency, since the device starts on the first relaunched kernel
+  // while the host enqueues the rest
   constexpr int kBatch = 1;
   volatile char* pin = (volatile char*)P.pinned + 512;  // slot s: KState copy at byte 512 + 128 s
   static_assert(sizeof(KState) <= 128, "KState must fit a 128-byte pinned slot");
@@ -292,15 +296,29 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
   std::vector<double> hist;  // rr after each polled batch
   std::vector<int> hist_k;
   if (!hs.done) {
-    enqueue(0);
-    int slot = 0;
-    int64_t launched = kBatch;
+    // at most two batches in flight, one per pinned slot: a slot is reused only after it is read
+    int next_slot = 0, read_slot = 0, in_flight = 0;
+    int64_t launched = 0;
+    bool hold = false;
+    double rr_prev = hs.rr;
     for (;;) {
-      enqueue(slot ^ 1);
-      launched += kBatch;
-      APX_HIP(hipEventSynchronize(evs[slot]));
-      std::memcpy(&hs, (const void*)(pin + 128 * slot), sizeof(KState));
+      while (in_flight < 2 && !(hold && in_flight > 0)) {
+        enqueue(next_slot);
+        next_slot ^= 1;
+        ++in_flight;
+        launched += kBatch;
+      }
+      APX_HIP(hipEventSynchronize(evs[read_slot]));
+      std::memcpy(&hs, (const void*)(pin + 128 * read_slot), sizeof(KState));
+      read_slot ^= 1;
+      --in_flight;
       if (hs.done) break;
+      // the queued batch ends the solve: predicted to converge (rr falling by its last ratio
+      // per iteration) or reaching maxiter
+      const double ratio = rr_prev > 0.0 ? hs.rr / rr_prev : 1.0;
+      hold = in_flight > 0 && ((ratio < 1.0 && hs.rr * std::pow(ratio, (double)kBatch) <= hs.atol2) ||
+                               hs.k + kBatch >= hs.maxiter);
+      rr_prev = hs.rr;
       hist.push_back(hs.rr);
       hist_k.push_back(hs.k);
       // stagnation: the residual norm has not reached a new minimum (by 1 %) for
@@ -317,7 +335,6 @@ FcgResult fcg_solve(Plan& P, UNet& N, const T* vals, const double* g_nodal, cons
       if (hs.rr > 1e8 * hs.rr0) res.stagnated = true;  // divergence
       if (!std::isfinite(hs.rr)) res.stagnated = true;
       if (res.stagnated || launched > (int64_t)maxiter + 4 * kBatch) break;
-      slot ^= 1;
     }
   }
   APX_HIP(hipStreamSynchronize(st));
