@@ -21,7 +21,8 @@ each method:
                   --workload c4 --n 2048 --save-net), applied with k = 1: a
                   network that never saw this operator (not in the default list)
 The network is trained on the GPU before the timed steps (learned.fit: pairs
-from plain Jacobi-PCG solves of random right-hand sides, A-norm loss); its
+from plain Jacobi-PCG solves of random right-hand sides, then also from the
+flexible-CG iterates of the network being trained, A-norm loss); its
 setup time is reported in `learned_setup` (with the time-to-solution including
 it and the number of solves after which the training has paid for itself)
 and is not part of any step.
