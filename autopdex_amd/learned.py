@@ -236,14 +236,17 @@ def default_levels(height, width, coarsest=3):
 
 
 def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=16, n_smooth=16,
-        pair_tol=1e-9, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
+        pair_tol=1e-7, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
         log=None, time_budget=None, jacobi_skip=False, self_fraction=1.0 / 3.0, self_lr=None,
         self_iterates=(4, 8, 16, 32)):
     """Train a linear U-Net preconditioner for `problem` (d DOFs per node on a structured
     height x width node grid: d input and output channels) and attach it.
 
-    Training pairs come from the plain solver: Jacobi-PCG solves of A e = r for white-noise
-    and smooth random right-hand sides r (squared-exponential fields of lengths 0.2 ... 0.005).
+    Training pairs come from the plain solver: Jacobi-PCG solves of A e = r, to a relative
+    residual of pair_tol, for white-noise and smooth random right-hand sides r
+    (squared-exponential fields of lengths 0.2 ... 0.005).  At 2048^2, pair_tol 1e-7 gave
+    28 / 28 / 38 flexible-CG iterations in the benchmark's learned_mixed solve for seeds 0 / 1 / 2,
+    against 31 / 32 / 40 with 1e-9, and 3-5 s less pair generation.
     The loss is the relative A-norm error, the quantity CG minimizes.  Every `validate_every`
     steps the parameters are scored by flexible CG (to 1e-6) on a held-out right-hand side and
     the best ones are kept.
@@ -252,9 +255,9 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     network meets: the best network so far runs `self_iterates` iterations of flexible CG on
     every training right-hand side b, and each iterate x_k adds the pair (b - A x_k, x* - x_k)
     (x* from the plain solve).  These steps run at `self_lr` (default 0.15 lr), decaying to a
-    third of it.  At 2048^2 this took the benchmark's learned_mixed Newton solve from 48 to 33
-    flexible-CG iterations at the same number of steps; Jacobi-PCG iterates as extra pairs made
-    the network fail instead.  self_fraction = 0 trains on the plain solver's pairs only.
+    third of it.  At 2048^2 (pair_tol 1e-9) this took the benchmark's learned_mixed Newton solve
+    from 48 to 31 flexible-CG iterations at the same number of steps (seed 1: 45 to 32); Jacobi-PCG
+    iterates as extra pairs made the network fail instead.  self_fraction = 0 trains on the plain solver's pairs only.
     Returns (network with batch 1, attached; info dict)."""
     torch = torch_cuda()
     t0 = time.perf_counter()
