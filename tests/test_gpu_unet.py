@@ -365,6 +365,9 @@ def test_learned_mixed_newton_reaches_fp64_accuracy(torch_gpu):
     n = 64
     dofs, settings, static, o, prob = _poisson_problem(n)
     net, info = learned.fit(prob, n + 1, n + 1, n_noise=4, n_smooth=4, steps=400, validate_every=200)
+    # the last third of the steps trained on pairs from the network's own flexible-CG iterates
+    print(f"[learned mixed] fit: {info['self_pairs']} flexible-CG pairs, {info['self_steps']} steps", flush=True)
+    assert info["self_steps"] == 133 and info["self_pairs"] > 0
     s = dict(static, **{"type of preconditioner": "learned", "hip precision": "mixed", "reliable delta": 1e-2,
                         "linear tolerance": 1e-10, "linear forcing": True, "newton tolerance mode": "relative",
                         "linear maxiter": 2000})
