@@ -259,6 +259,8 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     from 48 to 31 flexible-CG iterations at the same number of steps (seed 1: 45 to 32); Jacobi-PCG
     iterates as extra pairs made the network fail instead.  self_fraction = 0 trains on the plain solver's pairs only.
     Returns (network with batch 1, attached; info dict)."""
+    if not 0.0 <= self_fraction < 1.0:
+        raise ValueError("self_fraction must be in [0, 1)")
     torch = torch_cuda()
     t0 = time.perf_counter()
     if vals is None:
@@ -307,8 +309,9 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
         if done == n_first and n_self > 0:
             # pairs from the flexible-CG iterates of the best network so far
             ts = time.perf_counter()
-            inf.set_params(best[1])
-            net.set_params(best[1])
+            theta = best[1] if best is not None else net.get_params()
+            inf.set_params(theta)
+            net.set_params(theta)
             # (x* solves to pair_tol: an iterate whose residual is near that carries no signal)
             for b, xs in pairs[:n_plain]:
                 for k_it in self_iterates:

@@ -105,3 +105,12 @@ def test_default_levels_reach_a_3x3_grid():
     assert learned.default_levels(33, 33) == 5
     assert learned.default_levels(513, 129) == 7      # limited by the shorter side
     assert learned.default_levels(3, 3) == 1
+
+
+@pytest.mark.parametrize("frac", [1.0, -0.1, 1.5])
+def test_fit_rejects_self_fraction_outside_unit_interval(frac):
+    """The flexible-CG pair phase takes a fraction in [0, 1) of the steps; anything else is
+    rejected before any device work (no problem object is needed to reach the check)."""
+    from autopdex_amd import learned
+    with pytest.raises(ValueError, match="self_fraction"):
+        learned.fit(None, 9, 9, self_fraction=frac)
