@@ -25,7 +25,7 @@ namespace apx {
 using namespace nn;
 
 namespace {
-constexpr int kWgradBlocks = 512;
+constexpr int kWgradBlocks = 2048;  // rows of the weight-gradient partial buffer
 
 int kblocks(int ci) { return ci >= 16 ? 9 * (ci / 16) : (9 * ci + 15) / 16; }
 
@@ -94,16 +94,19 @@ void launch_conv_ws(const ConvArgs& a, hipStream_t st) {
   launch_conv_ws_t<CI, CO, 4>(a, st);
 }
 
-template <int CI, int CO>
-void launch_wgrad_t(const WgradArgs& a, hipStream_t st) {
+// weight gradient: the blocks the device holds at once (at most kWgradBlocks, the partial
+// buffer's rows); returns the number launched, whose partials k_reduce_wgrad sums
+template <int CI, int CO, bool YM>
+int launch_wgrad_t(const WgradArgs& a, hipStream_t st) {
   const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kWgradBlocks));
-  hipLaunchKernelGGL((k_conv3x3_wgrad<CI, CO>), dim3(g), dim3(kConvThreads), 0, st, a);
+  static const int64_t cap = std::min<int64_t>(resident_blocks(k_conv3x3_wgrad<CI, CO, YM>), kWgradBlocks);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+  hipLaunchKernelGGL((k_conv3x3_wgrad<CI, CO, YM>), dim3(g), dim3(kConvThreads), 0, st, a);
+  return g;
 }
-
-int wgrad_grid(const WgradArgs& a) {
-  const int64_t tiles = (int64_t)a.B * ((a.W + kTileW - 1) / kTileW) * ((a.H + kTileH - 1) / kTileH);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kWgradBlocks));
+template <int CI, int CO>
+int launch_wgrad_t(const WgradArgs& a, hipStream_t st) {
+  return a.ymask ? launch_wgrad_t<CI, CO, true>(a, st) : launch_wgrad_t<CI, CO, false>(a, st);
 }
 
 #define APX_CONV_CASES(X) \
@@ -142,7 +145,7 @@ void launch_conv(int ci, int co, bool f32, const ConvArgs& a, hipStream_t st) {
                                        " channels");
 }
 
-void launch_wgrad(int ci, int co, const WgradArgs& a, hipStream_t st) {
+int launch_wgrad(int ci, int co, const WgradArgs& a, hipStream_t st) {
 #define APX_X(A, B) \
   if (ci == A && co == B) return launch_wgrad_t<A, B>(a, st);
   APX_CONV_CASES(APX_X)
@@ -575,17 +578,30 @@ __global__ void k_flip_weights(const float* theta, uint16_t* wd, int64_t w_off, 
   }
 }
 
-__global__ void k_reduce_wgrad(const float* partial, int nblk, int64_t stride, int cop, int wk, int co_real,
-                               float* grad, int64_t w_off, int64_t b_off) {
+// the weight-gradient partials summed over the nblk blocks (row b at b * stride: COP*WK weight
+// entries, then the biases): 64 outputs per block, each of its 16 waves summing a contiguous
+// range of the rows in order, the 16 range sums added in order (deterministic)
+constexpr int kReduceWaves = 16;
+__global__ __launch_bounds__(64 * kReduceWaves) void k_reduce_wgrad(const float* partial, int nblk, int64_t stride,
+                                                                    int cop, int wk, int co_real, float* grad,
+                                                                    int64_t w_off, int64_t b_off) {
+  __shared__ float red[kReduceWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t n = (int64_t)cop * wk + co_real;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t src = i < (int64_t)cop * wk ? i : (int64_t)cop * wk + (i - (int64_t)cop * wk);
-    float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * stride + src];
+  const int64_t i = blockIdx.x * 64LL + lane;
+  const int per = (nblk + kReduceWaves - 1) / kReduceWaves, b0 = w * per, b1 = min(nblk, b0 + per);
+  float s = 0.f;
+  if (i < n)
+    for (int b = b0; b < b1; ++b) s += partial[(int64_t)b * stride + i];
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && i < n) {
+    float t = 0.f;
+    for (int k = 0; k < kReduceWaves; ++k) t += red[k][lane];
     if (i < (int64_t)cop * wk)
-      grad[w_off + i] = s;
+      grad[w_off + i] = t;
     else
-      grad[b_off + (i - (int64_t)cop * wk)] = s;
+      grad[b_off + (i - (int64_t)cop * wk)] = t;
   }
 }
 
@@ -892,12 +908,10 @@ void conv_wgrad(UNet& N, int li, int level, const uint16_t* x, const uint16_t* d
   a.B = N.batch;
   a.H = N.H[level];
   a.W = N.W[level];
-  launch_wgrad(ly.ci, ly.co, a, st);
-  const int nblk = wgrad_grid(a);
-  int g;
-  grid1d((int64_t)ly.cop * ly.wk + ly.co, g);
-  hipLaunchKernelGGL(k_reduce_wgrad, dim3(g), dim3(256), 0, st, N.wpart.p, nblk, (int64_t)ly.cop * ly.wk + ly.cop,
-                     ly.cop, ly.wk, ly.co, N.grad.p, ly.w_off, ly.b_off);
+  const int nblk = launch_wgrad(ly.ci, ly.co, a, st);
+  const int64_t n_out = (int64_t)ly.cop * ly.wk + ly.co;
+  hipLaunchKernelGGL(k_reduce_wgrad, dim3((unsigned)((n_out + 63) / 64)), dim3(64 * kReduceWaves), 0, st, N.wpart.p,
+                     nblk, (int64_t)ly.cop * ly.wk + ly.cop, ly.cop, ly.wk, ly.co, N.grad.p, ly.w_off, ly.b_off);
 }
 
 // d_0 = conv_e[0](conv_in(x0)) without storing conv_in's output: the input layer is formed while

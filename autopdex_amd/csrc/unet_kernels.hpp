@@ -969,19 +969,31 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_in(ConvArgs a) {
 }
 
 // ---- weight gradient ---------------------------------------------------------
-// dW[t, co, ci] = sum_p dY[p, co] X[p + o_t, ci]   (dY masked by ymask > 0 when given)
+// dW[t, co, ci] = sum_p dY[p, co] X[p + o_t, ci]   (dY masked by ymask > 0 when YM)
 // db[co] = sum_p dY[p, co].  Each block accumulates over its tiles and writes
-// one partial [KB*16 x COP] + [COP] in FP32; k_reduce_partials sums the blocks
-// in index order.  MFMA: A = dY^T (16 co x 16 px), B = X_t (16 px x 16 k).
+// one partial [KB*16 x COP] + [COP] in FP32; k_reduce_wgrad sums the blocks
+// in a fixed order.  MFMA: A = dY^T (16 co x 16 px), B = X_t (16 px x 16 k).
+// A tile's loads (the X tile with its one-pixel halo and the dY tile; 8 channels, or one
+// pixel of at most 2 channels, per load) are all in flight at once: every address is clamped
+// into the image and what lies outside is stored as zeros, so that no load sits under a
+// per-lane condition (the compiler would wait for each at its branch join).
 struct WgradArgs {
   const uint16_t* x;      // B x H x W x CI
   const uint16_t* dy;     // B x H x W x CO
-  const uint16_t* ymask;  // optional (ReLU output of the layer)
+  const uint16_t* ymask;  // ReLU output of the layer (YM only)
   float* partial;         // gridDim x (COP*WK + COP)
   int B, H, W;
 };
 
-template <int CI, int CO>
+// zero the 16-bit halves of q whose mask half is not > 0 (ReLU backward)
+__device__ __forceinline__ uint32_t relu_keep(uint32_t q, uint32_t m) {
+  uint32_t keep = 0u;
+  if (bf2f((uint16_t)(m & 0xffffu)) > 0.f) keep |= 0xffffu;
+  if (bf2f((uint16_t)(m >> 16)) > 0.f) keep |= 0xffff0000u;
+  return q & keep;
+}
+
+template <int CI, int CO, bool YM>
 __global__ __launch_bounds__(kConvThreads) void k_conv3x3_wgrad(WgradArgs a) {
   constexpr int S = lds_stride(CI);
   constexpr int COP = round16(CO);
@@ -992,47 +1004,112 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_wgrad(WgradArgs a) {
   // fragments (co tile n, k block kb) are distributed over the 4 wavefronts
   constexpr int NF = NT * KB;
   constexpr int FPW = (NF + kTileH - 1) / kTileH;
-  __shared__ uint16_t sx[(kTileH + 2) * (kTileW + 2) * S];
-  __shared__ uint16_t sd[kTileH * kTileW * SD];
+  static_assert(CI % 8 == 0 || CI <= 2, "X loads: 8 channels, or one pixel of at most 2");
+  static_assert(CO % 16 == 0 || CO <= 2, "dY loads: 8 channels, or one pixel of at most 2");
+  constexpr int XL = CI % 8 == 0 ? CI / 8 : 1, DL = CO % 8 == 0 ? CO / 8 : 1;  // loads per pixel
+  constexpr int NPX = (kTileH + 2) * (kTileW + 2), NPD = kTileH * kTileW;       // staged pixels
+  constexpr int NXL = NPX * XL, NDL = NPD * DL;
+  constexpr int RX = (NXL + kConvThreads - 1) / kConvThreads, RD = (NDL + kConvThreads - 1) / kConvThreads;
+  static_assert(RX <= 32 && RD <= 32, "one validity bit per load");
+  // LDS stores of 8 channels are two 8-byte writes: pixel strides are multiples of 4 elements
+  static_assert(S % 4 == 0 && SD % 4 == 0, "8-byte aligned LDS chunks");
+  __shared__ __attribute__((aligned(16))) uint16_t sx[NPX * S];
+  __shared__ __attribute__((aligned(16))) uint16_t sd[NPD * SD];
   __shared__ float sbias[kTileH][COP];
   const int tiles_x = (a.W + kTileW - 1) / kTileW, tiles_y = (a.H + kTileH - 1) / kTileH;
   const int64_t ntiles = (int64_t)a.B * tiles_x * tiles_y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lg = lane >> 4;
+  // dY channels CO .. COP - 1 (CO <= 2) are read as zeros: the staging writes channels < CO only
+  if constexpr (CO < COP)
+    for (int i = threadIdx.x; i < NPD * (COP - CO); i += kConvThreads)
+      sd[(i / (COP - CO)) * SD + CO + i % (COP - CO)] = 0;
   f32x4 acc[FPW];
 #pragma unroll
   for (int f = 0; f < FPW; ++f) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
   float bacc = 0.f;  // bias partial: thread -> (row = wave, co = lane) when lane < COP
+  u32x4 qx[RX], qd[RD];
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int img = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile % ((int64_t)tiles_x * tiles_y));
     const int ty0 = (trem / tiles_x) * kTileH, tx0 = (trem % tiles_x) * kTileW;
-    __syncthreads();
-    {
-      constexpr int NP = (kTileH + 2) * (kTileW + 2);
-      const uint16_t* xb = a.x + (int64_t)img * a.H * a.W * CI;
-      for (int i = threadIdx.x; i < NP * S; i += kConvThreads) {
-        const int pix = i / S, c = i % S;
-        const int yy = ty0 - 1 + pix / (kTileW + 2), xx = tx0 - 1 + pix % (kTileW + 2);
-        uint16_t v = 0;
-        if (c < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = xb[((int64_t)yy * a.W + xx) * CI + c];
-        sx[i] = v;
-      }
-      const uint16_t* db = a.dy + (int64_t)img * a.H * a.W * CO;
-      const uint16_t* mb = a.ymask ? a.ymask + (int64_t)img * a.H * a.W * CO : nullptr;
-      for (int i = threadIdx.x; i < kTileH * kTileW * SD; i += kConvThreads) {
-        const int pix = i / SD, c = i % SD;
-        const int yy = ty0 + pix / kTileW, xx = tx0 + pix % kTileW;
-        uint16_t v = 0;
-        if (c < CO && yy < a.H && xx < a.W) {
-          const int64_t o = ((int64_t)yy * a.W + xx) * CO + c;
-          v = db[o];
-          if (mb && !(bf2f(mb[o]) > 0.f)) v = 0;
+    const int64_t pix0 = (int64_t)img * a.H * a.W;
+    // loads into registers; bit r of inx / ind: load r lies inside the tile's arrays and image
+    uint32_t inx = 0, ind = 0;
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+      const int i0 = (int)threadIdx.x + r * kConvThreads, i = min(i0, NXL - 1);
+      const int pix = i / XL, h = i % XL;
+      const int yy = ty0 - 1 + pix / (kTileW + 2), xx = tx0 - 1 + pix % (kTileW + 2);
+      inx |= (uint32_t)(i0 < NXL && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) << r;
+      const int yc = min(max(yy, 0), a.H - 1), xc = min(max(xx, 0), a.W - 1);
+      const uint16_t* src = a.x + (pix0 + (int64_t)yc * a.W + xc) * CI + h * 8;
+      if constexpr (CI % 8 == 0)
+        qx[r] = *(const u32x4*)src;
+      else if constexpr (CI == 2)
+        qx[r] = u32x4{*(const uint32_t*)src, 0u, 0u, 0u};
+      else
+        qx[r] = u32x4{(uint32_t)*src, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int i0 = (int)threadIdx.x + r * kConvThreads, i = min(i0, NDL - 1);
+      const int pix = i / DL, h = i % DL;
+      const int yy = ty0 + pix / kTileW, xx = tx0 + pix % kTileW;
+      ind |= (uint32_t)(i0 < NDL && yy < a.H && xx < a.W) << r;
+      const int yc = min(yy, a.H - 1), xc = min(xx, a.W - 1);
+      const int64_t o = (pix0 + (int64_t)yc * a.W + xc) * CO + h * 8;
+      u32x4 q;
+      if constexpr (CO % 8 == 0) {
+        q = *(const u32x4*)(a.dy + o);
+        if constexpr (YM) {
+          const u32x4 m = *(const u32x4*)(a.ymask + o);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) q[e] = relu_keep(q[e], m[e]);
         }
-        sd[i] = v;
+      } else if constexpr (CO == 2) {
+        q = u32x4{*(const uint32_t*)(a.dy + o), 0u, 0u, 0u};
+        if constexpr (YM) q[0] = relu_keep(q[0], *(const uint32_t*)(a.ymask + o));
+      } else {
+        q = u32x4{(uint32_t)a.dy[o], 0u, 0u, 0u};
+        if constexpr (YM) q[0] = relu_keep(q[0], (uint32_t)a.ymask[o]);
+      }
+      qd[r] = q;
+    }
+    lds_barrier();  // every wave is done with the previous tile's LDS
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+      const int i = (int)threadIdx.x + r * kConvThreads;
+      if (i < NXL) {
+        const u32x4 q = ((inx >> r) & 1u) ? qx[r] : u32x4{0u, 0u, 0u, 0u};
+        uint16_t* d = sx + (i / XL) * S + (i % XL) * 8;
+        if constexpr (CI % 8 == 0) {
+          *(uint2*)d = uint2{q[0], q[1]};
+          *(uint2*)(d + 4) = uint2{q[2], q[3]};
+        } else if constexpr (CI == 2) {
+          *(uint32_t*)d = q[0];
+        } else {
+          *d = (uint16_t)q[0];
+        }
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int i = (int)threadIdx.x + r * kConvThreads;
+      if (i < NDL) {
+        const u32x4 q = ((ind >> r) & 1u) ? qd[r] : u32x4{0u, 0u, 0u, 0u};
+        uint16_t* d = sd + (i / DL) * SD + (i % DL) * 8;
+        if constexpr (CO % 8 == 0) {
+          *(uint2*)d = uint2{q[0], q[1]};
+          *(uint2*)(d + 4) = uint2{q[2], q[3]};
+        } else if constexpr (CO == 2) {
+          *(uint32_t*)d = q[0];
+        } else {
+          *d = (uint16_t)q[0];
+        }
+      }
+    }
+    lds_barrier();
     if (lane < COP) {
       float s = 0.f;
       for (int px = 0; px < kTileW; ++px) s += bf2f(sd[(wave * kTileW + px) * SD + lane]);
