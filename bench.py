@@ -47,8 +47,9 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--aux-steps 3] [--n 2048]
                 [--headline learned_mixed] [--train-steps 1500] [--no-cpu]
 python bench.py --workload c4 [--fields 2000] [--heldout 200] [--train-steps 8000]
                 (the learned preconditioner on held-out variable-coefficient fields at 1024^2)
-python bench.py --probe [--workload c3]
-                (bounded kernel probes for rocprofv3 counter runs: 2048^2 Poisson, 64^3 Hex8)
+python bench.py --probe [--workload c3|train]
+                (bounded kernel probes for rocprofv3 runs: 2048^2 Poisson, 64^3 Hex8, 60 training
+                steps of the 2048^2 network)
 """
 import argparse
 import ctypes as C
@@ -336,6 +337,33 @@ def build_cube_problem(n, lam=10.0, mu=1.0, disp=0.02):
     return backend.get_problem(np.zeros(mask.shape), settings, static), static, values
 
 
+def probe_train(args):
+    """Bounded training probe for kernel traces: the benchmark's network (batch 4) on 4 pairs
+    (white-noise right-hand sides, plain Jacobi-PCG to 1e-3), 5 untimed and 60 timed
+    learned.train steps at a fixed learning rate; prints the wall time per step."""
+    import torch
+    from autopdex_amd import learned
+    prob = build_problem(args.n)[0]
+    vals = learned.tangent(prob)
+    gen = torch.Generator(device=prob.device)
+    gen.manual_seed(args.seed)
+    pairs = []
+    for _ in range(4):
+        b = torch.randn(prob.n_free, generator=gen, device=prob.device, dtype=torch.float64)
+        pairs += learned.solver_pairs(prob, vals, b, checkpoints=(0,), tol=1e-3)
+    H = args.n + 1
+    L = learned.default_levels(H, H)
+    net = learned.UNetPreconditioner(H, H, L, ((16, 32) + (64,) * 10)[:L], 1, 1, relu=False, batch=4, seed=1)
+    net.set_params(net.initial_params(1, up_scale=0.05))
+    learned.train(net, prob, pairs, vals, steps=5, lr=1e-5, lr_final=1e-5, log=None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    learned.train(net, prob, pairs, vals, steps=60, lr=1e-5, lr_final=1e-5, log=None)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / 60 * 1e3
+    print(json.dumps({"probe": "train", "n": args.n, "batch": 4, "steps": 60, "ms_per_step": ms}))
+
+
 def probe_cube(args):
     """The 64^3 Hex8 Neo-Hooke configuration's kernels (colour-pass assembly, 3-DOF SpMV), for
     kernel-trace and counter runs: one Newton iteration of 20 Jacobi-PCG iterations from u = 0
@@ -510,10 +538,10 @@ def main():
                          "--save-net) applied with k = 1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="bounded kernel probe for counter runs")
-    ap.add_argument("--workload", default="poisson", choices=("poisson", "c3", "c4"),
+    ap.add_argument("--workload", default="poisson", choices=("poisson", "c3", "c4", "train"),
                     help="poisson: the 2048^2 time-to-solution benchmark; c4: the learned preconditioner on "
                          "held-out variable-coefficient fields at 1024^2; c3 (with --probe): the 64^3 Hex8 "
-                         "Neo-Hooke kernels")
+                         "Neo-Hooke kernels; train (with --probe): 60 training steps of the 2048^2 network")
     ap.add_argument("--fields", type=int, default=2000, help="c4: training fields")
     ap.add_argument("--heldout", type=int, default=200, help="c4: held-out fields")
     args = ap.parse_args()
@@ -522,7 +550,9 @@ def main():
     if args.train_steps is None:
         args.train_steps = 8000 if args.workload == "c4" else 1500
     if args.probe:
-        return probe_cube(args) if args.workload == "c3" else probe(args)
+        return {"c3": probe_cube, "train": probe_train}.get(args.workload, probe)(args)
+    if args.workload == "train":
+        raise SystemExit("--workload train is a probe: use it with --probe")
     if args.workload == "c4":
         return c4_workload(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
