@@ -238,7 +238,7 @@ def default_levels(height, width, coarsest=3):
 def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=16, n_smooth=16,
         pair_tol=1e-7, steps=1500, batch=4, lr=1e-4, up_scale=0.05, seed=0, validate_every=250,
         log=None, time_budget=None, jacobi_skip=False, self_fraction=1.0 / 3.0, self_lr=None,
-        self_iterates=(4, 8, 16, 32)):
+        self_iterates=(4, 8, 16, 32), bootstrap=None):
     """Train a linear U-Net preconditioner for `problem` (d DOFs per node on a structured
     height x width node grid: d input and output channels) and attach it.
 
@@ -257,7 +257,15 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     (x* from the plain solve).  These steps run at `self_lr` (default 0.15 lr), decaying to a
     third of it.  At 2048^2 (pair_tol 1e-9) this took the benchmark's learned_mixed Newton solve
     from 48 to 31 flexible-CG iterations at the same number of steps (seed 1: 45 to 32); Jacobi-PCG
-    iterates as extra pairs made the network fail instead.  self_fraction = 0 trains on the plain solver's pairs only.
+    iterates as extra pairs made the network fail instead.  self_fraction = 0 trains on the
+    plain solver's pairs only.
+
+    bootstrap = (n, k): only n right-hand sides (half white noise, half smooth) are solved by
+    Jacobi-PCG up front; after k steps the best network so far solves the others by flexible CG
+    to pair_tol (Jacobi-PCG finishes a solve that stagnates), which is far cheaper than plain
+    solves, and they join the pairs.  At 2048^2, (8, 500) cut the setup from 48 to 32 s for one
+    more flexible-CG iteration in learned_mixed (29 against 28, seed 0); (8, 250): 29 / 29 / 40
+    iterations for seeds 0 / 1 / 2.  None (the default) solves every right-hand side up front.
     Returns (network with batch 1, attached; info dict)."""
     if not 0.0 <= self_fraction < 1.0:
         raise ValueError("self_fraction must be in [0, 1)")
@@ -283,15 +291,21 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
         return torch.stack(comps, dim=1).reshape(-1)[free].contiguous()
 
     rhs = [noise() for _ in range(n_noise)] + [smooth(s) for s in range(n_smooth)]
-    pairs = []
-    for b in rhs:
-        pairs += solver_pairs(problem, vals, b, checkpoints=(0,), tol=pair_tol)
-    n_plain = len(pairs)  # pairs[i] = (b_i, x*_i): the iterate x_0 = 0
+    if bootstrap:
+        nb, boot_steps = int(bootstrap[0]), int(bootstrap[1])
+        first = list(range(min(nb // 2, n_noise))) + list(range(n_noise, n_noise + min(nb - nb // 2, n_smooth)))
+    else:
+        boot_steps, first = -1, list(range(len(rhs)))
+    later = [i for i in range(len(rhs)) if i not in first]
+    base = []  # (b, x*): the pair of the iterate x_0 = 0 of each right-hand side solved so far
+    for i in first:
+        base += solver_pairs(problem, vals, rhs[i], checkpoints=(0,), tol=pair_tol)
+    pairs = list(base)
     b_val = noise() + smooth(n_smooth + 1) * 10.0  # held out
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
     if log:
-        log(f"[fit] {len(pairs)} (r, A^-1 r) pairs from {len(rhs)} plain solves in {t_pairs:.1f} s; levels {L}, "
+        log(f"[fit] {len(pairs)} (r, A^-1 r) pairs from {len(first)} plain solves in {t_pairs:.1f} s; levels {L}, "
             f"channels {ch}")
     net = UNetPreconditioner(height, width, L, ch, d, d, relu=False, batch=batch, seed=seed + 1,
                              jacobi_skip=jacobi_skip)
@@ -304,8 +318,29 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     n_self = int(round(steps * self_fraction)) if self_iterates else 0
     n_first = steps - n_self
     lr_self = 0.15 * lr if self_lr is None else self_lr
-    n_self_pairs, t_self = 0, 0.0
+    n_self_pairs, t_self, n_boot, t_boot = 0, 0.0, len(later), 0.0
     while done < steps:
+        if done == boot_steps and later:
+            # the remaining right-hand sides, solved by flexible CG with the best network so far
+            tb = time.perf_counter()
+            inf.set_params(best[1] if best is not None else net.get_params())
+            for i in later:
+                b = rhs[i]
+                x, ks = fcg(problem, vals, b, tol=pair_tol, maxiter=2000)
+                if not bool(torch.isfinite(x).all()):
+                    x = torch.zeros_like(b)
+                if not ks.converged:  # stagnation: Jacobi-PCG finishes from the iterate
+                    r2 = b - spmv(problem, vals, x)
+                    rel = pair_tol * float(b.norm()) / max(float(r2.norm()), 1e-300)
+                    x2, _ = pcg(problem, vals, r2, tol=min(rel, 1.0))
+                    x = x + x2
+                base.append((b, x))
+                pairs.append((b, x))
+            later = []
+            torch.cuda.synchronize()
+            t_boot = time.perf_counter() - tb
+            if log:
+                log(f"[fit] {n_boot} pairs from flexible-CG solves in {t_boot:.1f} s")
         if done == n_first and n_self > 0:
             # pairs from the flexible-CG iterates of the best network so far
             ts = time.perf_counter()
@@ -313,7 +348,7 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
             inf.set_params(theta)
             net.set_params(theta)
             # (x* solves to pair_tol: an iterate whose residual is near that carries no signal)
-            for b, xs in pairs[:n_plain]:
+            for b, xs in base:
                 for k_it in self_iterates:
                     xk, _ = fcg(problem, vals, b, tol=0.0, maxiter=k_it)
                     rk = b - spmv(problem, vals, xk)
@@ -325,7 +360,7 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
             if log:
                 log(f"[fit] {n_self_pairs} pairs from flexible-CG iterates in {t_self:.1f} s")
         if done < n_first:
-            k = min(seg, n_first - done)
+            k = min(seg, (boot_steps if 0 <= done < boot_steps else n_first) - done)
             lr0 = lr * (0.1 ** (done / steps))
             lr1 = lr * (0.1 ** ((done + k) / steps))
         else:
@@ -349,7 +384,8 @@ def fit(problem, height, width, vals=None, levels=None, channels=None, n_noise=1
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     inf.set_params(best[1])
-    info = dict(pairs=n_plain, self_pairs=n_self_pairs, pair_seconds=t_pairs, self_pair_seconds=t_self,
+    info = dict(pairs=len(base) - n_boot, boot_pairs=n_boot, boot_pair_seconds=t_boot, self_pairs=n_self_pairs,
+                pair_seconds=t_pairs, self_pair_seconds=t_self,
                 train_seconds=t_train, steps=done, self_steps=max(0, done - n_first), best_step=best[2],
                 validation_iterations=best[0], levels=L, channels=list(ch), batch=batch,
                 final_loss=float(np.mean(history[-50:])), n_params=int(inf.n_params))
