@@ -1115,18 +1115,29 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3x3_wgrad(WgradArgs a) {
       for (int px = 0; px < kTileW; ++px) s += bf2f(sd[(wave * kTileW + px) * SD + lane]);
       bacc += s;
     }
-    // K = pixels of the tile: 4 rows x 64 px = 16 blocks of 16 px
+    // K = pixels of the tile: 4 rows x 64 px = 16 blocks of 16 px.  A wave's fragments cover at
+    // most two co tiles, n0 and n0 + 1 (FPW <= KB), so their A fragments are read once per pixel
+    // block and selected per fragment (the compiler cannot see that fid / KB is constant over f)
+    constexpr int kAN = 2;
+    static_assert(FPW <= KB, "a wave's fragments span at most two co tiles");
+    const int n0 = min(wave * FPW / KB, NT - 1);
     for (int pb = 0; pb < kTileH * (kTileW / 16); ++pb) {
       const int row = pb / (kTileW / 16), px0 = (pb % (kTileW / 16)) * 16;
       // A fragment: A[co = n*16 + lr][px = px0 + 4 lg + j] = dY[px][co]
+      bf16x4 an[kAN];
+#pragma unroll
+      for (int q = 0; q < kAN; ++q) {
+        const int nq = min(n0 + q, NT - 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) an[q][j] = (short)sd[(row * kTileW + px0 + 4 * lg + j) * SD + nq * 16 + lr];
+      }
 #pragma unroll
       for (int f = 0; f < FPW; ++f) {
         const int fid = wave * FPW + f;
         if (fid < NF) {
         const int n = fid / KB, kb = fid % KB;
-        bf16x4 af, bf;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) af[j] = (short)sd[(row * kTileW + px0 + 4 * lg + j) * SD + n * 16 + lr];
+        bf16x4 bf;
+        const bf16x4 af = n == n0 ? an[0] : an[kAN - 1];
         // B fragment: B[px = px0 + 4 lg + j][k = kb*16 + lr] = X[px + o_t][ci]
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
