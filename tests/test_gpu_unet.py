@@ -1,9 +1,9 @@
 """GPU tests of the learned preconditioner.
 
 - forward and parameter gradients of the HIP U-Net against the numpy oracle
-  (bf16 activations emulated in the oracle; tolerance: relative L2 error 2e-2
-  for outputs, 5e-2 per layer for gradients -- accumulation order and bf16
-  rounding-boundary effects only);
+  (bf16 activations emulated in the oracle; tolerance: relative L2 error 1.5e-2
+  for outputs, 1e-3 for the loss, 1e-2 per layer for gradients -- accumulation
+  order and bf16 rounding-boundary effects only);
 - the coarse levels fused into one launch give the same bits as one launch per
   layer (outputs and gradients, and the inference path), and match the oracle;
 - a network trained on the GPU on pairs from the plain solver needs fewer
@@ -149,13 +149,20 @@ def test_fused_coarse_tail_is_bit_identical_to_per_layer_launches(torch_gpu, H, 
         orc = OU.UNetOracle(L, ch, 1, 1, bool(relu), H, W, bf16_storage=True)
         y_ref = orc.forward(theta.astype(np.float64), OU.bf16(x))
         y1 = y1.reshape(B, H, W, 1)
-        assert np.linalg.norm(y1 - y_ref) / np.linalg.norm(y_ref) < 2e-2
+        e_y = np.linalg.norm(y1 - y_ref) / np.linalg.norm(y_ref)
         loss_ref, g_ref = orc.loss_backward(t.cpu().numpy().astype(np.float64).reshape(B, H, W, 1))
-        assert abs(l1 - loss_ref) / loss_ref < 2e-2
+        e_loss = abs(l1 - loss_ref) / loss_ref
+        errs = {}
         for name, (w_off, b_off, ci, co, cop, wk) in orc.layout.items():
             gw, gw_ref = g1[w_off:b_off + co], g_ref[w_off:b_off + co]
-            err = np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30)
-            assert err < 5e-2, (name, err)
+            errs[name] = np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30)
+        worst = max(errs, key=errs.get)
+        print(f"[fused tail vs oracle] y {e_y:.2e} loss {e_loss:.2e} worst layer {worst} {errs[worst]:.2e}", flush=True)
+        # measured: output 1.5e-3, loss 1.5e-5, worst layer gradient 3.7e-3 (down2); the bounds of
+        # the per-layer test above
+        assert e_y < 1.5e-2
+        assert e_loss < 1e-3
+        assert errs[worst] < 1e-2, (worst, errs[worst])
 
 
 @pytest.mark.parametrize("n,relu,c_in", [(128, 0, 1), (100, 1, 2), (2048, 0, 1)])
